@@ -1,0 +1,135 @@
+/*
+ * symforce_amd C ABI: the drop-in boundary between the host optimizer surface (sym::Optimizer,
+ * sym::LevenbergMarquardtSolver, sym::Factor, sym::Values — see include/sym/) and the MI355X HIP
+ * kernels. Plain pointers and sizes only; every call returns 0 on success and a negative code on
+ * failure, with the message available from symhip_last_error(). Host buffers are caller-owned;
+ * device buffers are owned by the handle. Handles are not thread safe (as the reference objects,
+ * opt/optimizer.h:23): use one per thread; each handle runs on its own HIP stream.
+ */
+#ifndef SYMHIP_H_
+#define SYMHIP_H_
+
+#include <stdint.h>
+
+#include "symhip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread. */
+const char* symhip_last_error(void);
+/* Number of visible devices (0 when no GPU / runtime unavailable). */
+int symhip_device_count(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Problem: keys, Values, factors and the linearization structure.
+ * Replaces the symbolic half of sym::Linearizer (opt/linearizer.cc:148-356): state index,
+ * hessian_lower pattern and per-factor storage maps, built once.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct symhip_problem* symhip_problem_t;
+
+int symhip_problem_create(const symhip_problem_desc* desc, symhip_problem_t* out);
+int symhip_problem_destroy(symhip_problem_t problem);
+/* out[0]=N (tangent dim), out[1]=M (residual dim), out[2]=nnz(hessian_lower),
+   out[3]=Values storage size, out[4]=number of optimized keys */
+int symhip_problem_dims(symhip_problem_t problem, int64_t* out5);
+/* CSC structure of hessian_lower (colptr N+1, rowind nnz). */
+int symhip_problem_hessian_structure(symhip_problem_t problem, int32_t* colptr, int32_t* rowind);
+/* Flattened hessian_storage_col_starts of every factor in factor order
+   (linearization_dense_key_helper_t, lcmtypes/symforce.lcm:67-91). Returns the count; pass
+   NULL to query it. */
+int64_t symhip_problem_col_starts(symhip_problem_t problem, int32_t* out);
+/* Optimized keys in state order and their tangent offsets (index_entry_t.offset). */
+int symhip_problem_state_keys(symhip_problem_t problem, int32_t* key_index, int32_t* tangent_offset);
+
+/* Linearizer::Relinearize (opt/linearizer.cc:54-120) at the given Values storage. Host buffers in
+   and out (copies over PCIe; the optimizer keeps everything resident instead). Outputs may be
+   NULL. hessian_values are in the reference CSC order. */
+int symhip_linearize(symhip_problem_t problem, const double* values, double* residual,
+                     double* hessian_values, double* rhs);
+
+
+/* ---------------------------------------------------------------------------------------------
+ * Optimizer: sym::Optimizer<double> + LevenbergMarquardtSolver with the linearization, linear
+ * solve, damping, retraction and accept/lambda logic on the device (opt/optimizer.tcc:21-89,
+ * opt/levenberg_marquardt_solver.tcc:138-343).
+ * ------------------------------------------------------------------------------------------- */
+enum symhip_linear_solver {
+  SYMHIP_SOLVER_CHOLESKY = 0, /* SparseCholeskySolver replacement: supernodal LDL^T on H */
+  SYMHIP_SOLVER_SCHUR = 1,    /* SparseSchurSolver replacement: landmark elimination + RCS */
+};
+enum symhip_ordering { SYMHIP_ORDERING_NATURAL = 0, SYMHIP_ORDERING_MIN_DEGREE = 1 };
+enum symhip_precision {
+  SYMHIP_PRECISION_F64 = 0,      /* plain FP64 everywhere */
+  SYMHIP_PRECISION_F32_JAC = 1,  /* FP32 Jacobians/Hessian blocks, FP64 residual/cost/decisions */
+};
+
+typedef struct symhip_solver_options {
+  int32_t linear_solver; /* symhip_linear_solver */
+  int32_t ordering;      /* symhip_ordering (fill-reducing ordering of the factorized matrix) */
+  int32_t precision;     /* symhip_precision */
+  int32_t use_pcg;       /* Schur path: preconditioned CG on the reduced camera system */
+  int32_t pcg_max_iters;
+  double pcg_rel_tol;
+} symhip_solver_options;
+
+typedef struct symhip_optimizer* symhip_optimizer_t;
+
+/* epsilon: the optimizer's epsilon (Optimizer ctor, default kDefaultEpsilon = 10 * DBL_EPSILON). */
+int symhip_optimizer_create(const symhip_problem_desc* desc, const symhip_optimizer_params* params,
+                            const symhip_solver_options* options, double epsilon,
+                            symhip_optimizer_t* out);
+int symhip_optimizer_destroy(symhip_optimizer_t opt);
+/* Optimizer::Optimize(values, num_iterations, populate_best_linearization, stats): values is
+   the full Values storage, updated in place with the best values. num_iterations < 0 uses
+   params.iterations. iters receives up to max_iters records (iteration -1 first). */
+int symhip_optimizer_optimize(symhip_optimizer_t opt, double* values, int32_t num_iterations,
+                              symhip_iteration_stats* iters, int32_t max_iters,
+                              symhip_optimization_stats* stats);
+/* Linearization of the best Values of the last optimize (stats.best_linearization). */
+int symhip_optimizer_best_linearization(symhip_optimizer_t opt, double* residual,
+                                        double* hessian_values, double* rhs);
+int symhip_optimizer_update_params(symhip_optimizer_t opt, const symhip_optimizer_params* params);
+/* Device-resident variant for throughput measurement: upload once, optimize without host
+   transfers of Values, read the best Values back afterwards. */
+int symhip_optimizer_set_values(symhip_optimizer_t opt, const double* values);
+int symhip_optimizer_optimize_resident(symhip_optimizer_t opt, int32_t num_iterations,
+                                       symhip_iteration_stats* iters, int32_t max_iters,
+                                       symhip_optimization_stats* stats);
+int symhip_optimizer_get_values(symhip_optimizer_t opt, double* values);
+/* Symbolic facts of the linear solver: out[0]=factorized dim, out[1]=supernodes,
+   out[2]=panel doubles, out[3]=levels, out[4]=factor flops (rounded). */
+int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out5);
+
+/* ---------------------------------------------------------------------------------------------
+ * Linear solver adapter: SparseCholeskySolver (opt/sparse_cholesky/sparse_cholesky_solver.h:
+ * 27-160) over a lower CSC matrix. node_ptr (n_nodes + 1 entries) groups consecutive scalars
+ * into nodes that are ordered together; NULL means one node per scalar.
+ * ordering: SYMHIP_ORDERING_*, or given_order (node elimination order) when non-NULL.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct symhip_cholesky* symhip_cholesky_t;
+
+int symhip_cholesky_create(int32_t n, const int32_t* colptr, const int32_t* rowind,
+                           const int32_t* node_ptr, int32_t n_nodes, int32_t ordering,
+                           const int32_t* given_order, symhip_cholesky_t* out);
+int symhip_cholesky_destroy(symhip_cholesky_t c);
+/* Factorize(A) with host values in the CSC order given at creation. */
+int symhip_cholesky_factorize(symhip_cholesky_t c, const double* values);
+/* Solve(b) for nrhs column-major right-hand sides (host buffers). */
+int symhip_cholesky_solve(symhip_cholesky_t c, const double* b, double* x, int32_t nrhs);
+/* inv_permutation (new -> old), the reference's Permutation().inverse() convention. */
+int symhip_cholesky_permutation(symhip_cholesky_t c, int32_t* inv_perm);
+/* Reference elimination tree parent and L column counts for the chosen permutation
+   (sparse_cholesky_solver.tcc:55-89). */
+int symhip_cholesky_etree(symhip_cholesky_t c, int32_t* parent, int32_t* counts);
+/* D of the last factorization (permuted order). */
+int symhip_cholesky_d(symhip_cholesky_t c, double* d);
+/* out[0]=n, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=factor flops. */
+int symhip_cholesky_info(symhip_cholesky_t c, int64_t* out5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMHIP_H_ */

@@ -282,3 +282,249 @@ def gen_pose_graph(n_nodes=100000, n_loops=30000, seed=42):
 def gen_localization(n_poses=20, n_landmarks=20, visible_fraction=0.3, seed=42):
     return _from_generated(
         problems_lib().symhip_gen_localization(n_poses, n_landmarks, visible_fraction, seed))
+
+
+# ---- the HIP library -----------------------------------------------------------------------
+_hip_lib = None
+
+
+def hip_lib():
+    """Loads libsymhip.so; raises if it has not been built (no fallback exists)."""
+    global _hip_lib
+    if _hip_lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m symforce_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.symhip_last_error.restype = C.c_char_p
+        L.symhip_device_count.restype = C.c_int
+        L.symhip_problem_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(vp)]
+        L.symhip_problem_destroy.argtypes = [vp]
+        L.symhip_problem_dims.argtypes = [vp, I64P]
+        L.symhip_problem_hessian_structure.argtypes = [vp, I32P, I32P]
+        L.symhip_problem_col_starts.argtypes = [vp, I32P]
+        L.symhip_problem_col_starts.restype = C.c_int64
+        L.symhip_problem_state_keys.argtypes = [vp, I32P, I32P]
+        L.symhip_linearize.argtypes = [vp, F64P, F64P, F64P, F64P]
+        for name, args, res in _EXTRA_SIGNATURES:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _hip_lib = L
+    return _hip_lib
+
+
+class SolverOptions(C.Structure):
+    _fields_ = [
+        ("linear_solver", C.c_int32),
+        ("ordering", C.c_int32),
+        ("precision", C.c_int32),
+        ("use_pcg", C.c_int32),
+        ("pcg_max_iters", C.c_int32),
+        ("pcg_rel_tol", C.c_double),
+    ]
+
+
+SOLVER_CHOLESKY, SOLVER_SCHUR = 0, 1
+ORDERING_NATURAL, ORDERING_MIN_DEGREE = 0, 1
+PRECISION_F64, PRECISION_F32_JAC = 0, 1
+
+_vp = C.c_void_p
+_EXTRA_SIGNATURES: list = [
+    ("symhip_optimizer_create", [C.POINTER(ProblemDesc), C.POINTER(OptimizerParams),
+                                 C.POINTER(SolverOptions), C.c_double, C.POINTER(_vp)], C.c_int),
+    ("symhip_optimizer_destroy", [_vp], C.c_int),
+    ("symhip_optimizer_optimize", [_vp, F64P, C.c_int32, C.POINTER(IterationStats), C.c_int32,
+                                   C.POINTER(OptimizationStats)], C.c_int),
+    ("symhip_optimizer_best_linearization", [_vp, F64P, F64P, F64P], C.c_int),
+    ("symhip_optimizer_update_params", [_vp, C.POINTER(OptimizerParams)], C.c_int),
+    ("symhip_optimizer_set_values", [_vp, F64P], C.c_int),
+    ("symhip_optimizer_optimize_resident", [_vp, C.c_int32, C.POINTER(IterationStats), C.c_int32,
+                                            C.POINTER(OptimizationStats)], C.c_int),
+    ("symhip_optimizer_get_values", [_vp, F64P], C.c_int),
+    ("symhip_optimizer_solver_info", [_vp, I64P], C.c_int),
+    ("symhip_cholesky_create", [C.c_int32, I32P, I32P, I32P, C.c_int32, C.c_int32, I32P,
+                                C.POINTER(_vp)], C.c_int),
+    ("symhip_cholesky_destroy", [_vp], C.c_int),
+    ("symhip_cholesky_factorize", [_vp, F64P], C.c_int),
+    ("symhip_cholesky_solve", [_vp, F64P, F64P, C.c_int32], C.c_int),
+    ("symhip_cholesky_permutation", [_vp, I32P], C.c_int),
+    ("symhip_cholesky_etree", [_vp, I32P, I32P], C.c_int),
+    ("symhip_cholesky_d", [_vp, F64P], C.c_int),
+    ("symhip_cholesky_info", [_vp, I64P], C.c_int),
+]
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(hip_lib().symhip_last_error().decode())
+
+
+class HipProblem:
+    """Problem handle of libsymhip (structure on the host, device state on first use)."""
+
+    def __init__(self, prob: Problem):
+        self.prob = prob
+        self._desc = prob.desc()
+        h = C.c_void_p()
+        check(hip_lib().symhip_problem_create(C.byref(self._desc), C.byref(h)))
+        self.h = h
+        dims = np.zeros(5, np.int64)
+        check(hip_lib().symhip_problem_dims(self.h, ptr(dims, C.c_int64)))
+        self.N, self.M, self.nnz, self.storage, self.n_state = (int(x) for x in dims)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hip_lib().symhip_problem_destroy(self.h)
+            self.h = None
+
+    def hessian_structure(self):
+        cp = np.zeros(self.N + 1, np.int32)
+        ri = np.zeros(self.nnz, np.int32)
+        check(hip_lib().symhip_problem_hessian_structure(self.h, ptr(cp, C.c_int32),
+                                                         ptr(ri, C.c_int32)))
+        return cp, ri
+
+    def col_starts(self):
+        n = hip_lib().symhip_problem_col_starts(self.h, None)
+        out = np.zeros(n, np.int32)
+        hip_lib().symhip_problem_col_starts(self.h, ptr(out, C.c_int32))
+        return out
+
+    def state_keys(self):
+        k = np.zeros(self.n_state, np.int32)
+        o = np.zeros(self.n_state, np.int32)
+        check(hip_lib().symhip_problem_state_keys(self.h, ptr(k, C.c_int32), ptr(o, C.c_int32)))
+        return k, o
+
+    def linearize(self, values=None):
+        v = self.prob.values if values is None else np.ascontiguousarray(values, np.float64)
+        r, h, b = np.zeros(self.M), np.zeros(self.nnz), np.zeros(self.N)
+        check(hip_lib().symhip_linearize(self.h, ptr(v, C.c_double), ptr(r, C.c_double),
+                                         ptr(h, C.c_double), ptr(b, C.c_double)))
+        return dict(residual=r, hessian_values=h, rhs=b)
+
+
+def solver_options(linear_solver=SOLVER_CHOLESKY, ordering=ORDERING_MIN_DEGREE,
+                   precision=PRECISION_F64, use_pcg=0, pcg_max_iters=100, pcg_rel_tol=1e-6):
+    return SolverOptions(linear_solver, ordering, precision, use_pcg, pcg_max_iters, pcg_rel_tol)
+
+
+class HipOptimizer:
+    """sym::Optimizer<double> over libsymhip (device-resident LM)."""
+
+    def __init__(self, prob: Problem, params: OptimizerParams | None = None, options=None,
+                 epsilon=None):
+        self.prob = prob
+        self.params = params or default_params()
+        self.options = options or solver_options()
+        eps = prob.info.get("epsilon", 10 * np.finfo(np.float64).eps) if epsilon is None else epsilon
+        self._desc = prob.desc()
+        h = C.c_void_p()
+        check(hip_lib().symhip_optimizer_create(C.byref(self._desc), C.byref(self.params),
+                                                C.byref(self.options), eps, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hip_lib().symhip_optimizer_destroy(self.h)
+            self.h = None
+
+    def optimize(self, values=None, num_iterations=-1):
+        vals = (self.prob.values if values is None else np.asarray(values, np.float64)).copy()
+        maxit = 4096
+        iters = (IterationStats * maxit)()
+        st = OptimizationStats()
+        check(hip_lib().symhip_optimizer_optimize(self.h, ptr(vals, C.c_double), num_iterations,
+                                                  iters, maxit, C.byref(st)))
+        return dict(values=vals, status=st.status, failure_reason=st.failure_reason,
+                    best_index=st.best_index,
+                    iterations=iteration_records(iters, st.num_iterations))
+
+    def set_values(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(hip_lib().symhip_optimizer_set_values(self.h, ptr(v, C.c_double)))
+
+    def optimize_resident(self, num_iterations=-1):
+        maxit = 4096
+        iters = (IterationStats * maxit)()
+        st = OptimizationStats()
+        check(hip_lib().symhip_optimizer_optimize_resident(self.h, num_iterations, iters, maxit,
+                                                           C.byref(st)))
+        return dict(status=st.status, failure_reason=st.failure_reason, best_index=st.best_index,
+                    iterations=iteration_records(iters, st.num_iterations))
+
+    def get_values(self):
+        v = np.zeros(self.prob.values.size)
+        check(hip_lib().symhip_optimizer_get_values(self.h, ptr(v, C.c_double)))
+        return v
+
+    def best_linearization(self, M, nnz, N):
+        r, h, b = np.zeros(M), np.zeros(nnz), np.zeros(N)
+        check(hip_lib().symhip_optimizer_best_linearization(
+            self.h, ptr(r, C.c_double), ptr(h, C.c_double), ptr(b, C.c_double)))
+        return dict(residual=r, hessian_values=h, rhs=b)
+
+    def solver_info(self):
+        out = np.zeros(5, np.int64)
+        check(hip_lib().symhip_optimizer_solver_info(self.h, ptr(out, C.c_int64)))
+        return dict(n=int(out[0]), supernodes=int(out[1]), panel=int(out[2]),
+                    levels=int(out[3]), flops=int(out[4]))
+
+
+class HipCholesky:
+    """SparseCholeskySolver replacement (supernodal LDL^T on the device)."""
+
+    def __init__(self, n, colptr, rowind, node_ptr=None, ordering=ORDERING_MIN_DEGREE,
+                 given_order=None):
+        self.n = n
+        self.colptr = np.ascontiguousarray(colptr, np.int32)
+        self.rowind = np.ascontiguousarray(rowind, np.int32)
+        npv = None if node_ptr is None else np.ascontiguousarray(node_ptr, np.int32)
+        gv = None if given_order is None else np.ascontiguousarray(given_order, np.int32)
+        h = C.c_void_p()
+        check(hip_lib().symhip_cholesky_create(
+            n, ptr(self.colptr, C.c_int32), ptr(self.rowind, C.c_int32),
+            None if npv is None else ptr(npv, C.c_int32), 0 if npv is None else npv.size - 1,
+            ordering, None if gv is None else ptr(gv, C.c_int32), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hip_lib().symhip_cholesky_destroy(self.h)
+            self.h = None
+
+    def inv_permutation(self):
+        out = np.zeros(self.n, np.int32)
+        check(hip_lib().symhip_cholesky_permutation(self.h, ptr(out, C.c_int32)))
+        return out
+
+    def etree(self):
+        p = np.zeros(self.n, np.int32)
+        c = np.zeros(self.n, np.int32)
+        check(hip_lib().symhip_cholesky_etree(self.h, ptr(p, C.c_int32), ptr(c, C.c_int32)))
+        return p, c
+
+    def info(self):
+        out = np.zeros(5, np.int64)
+        check(hip_lib().symhip_cholesky_info(self.h, ptr(out, C.c_int64)))
+        return dict(n=int(out[0]), supernodes=int(out[1]), panel=int(out[2]),
+                    levels=int(out[3]), flops=int(out[4]))
+
+    def factorize(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(hip_lib().symhip_cholesky_factorize(self.h, ptr(v, C.c_double)))
+
+    def solve(self, b):
+        b = np.asarray(b, np.float64)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        bb = np.ascontiguousarray(b.T if b.ndim == 2 else b)
+        x = np.zeros_like(bb)
+        check(hip_lib().symhip_cholesky_solve(self.h, ptr(bb, C.c_double), ptr(x, C.c_double),
+                                              nrhs))
+        return x.T if b.ndim == 2 else x
+
+    def d(self):
+        out = np.zeros(self.n)
+        check(hip_lib().symhip_cholesky_d(self.h, ptr(out, C.c_double)))
+        return out

@@ -1,0 +1,309 @@
+// extern "C" entry points of libsymhip (include/symhip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/symhip.h"
+#include "./common.h"
+#include "./chol_numeric.h"
+#include "./linearize.h"
+#include "./optimizer.h"
+#include "./problem.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return -1;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return -1;
+  }
+}
+}  // namespace
+
+struct symhip_problem {
+  symhip::HostProblem host;
+  hipStream_t stream = nullptr;
+  symhip::CscLinearizer lin;
+  symhip::DBuf<double> values, residual, hval, rhs;
+  bool device_ready = false;
+  // Device state is created on first use so the structure queries work without a GPU.
+  void ensure_device() {
+    if (device_ready) return;
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    lin.setup(host, stream);
+    device_ready = true;
+  }
+  ~symhip_problem() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct symhip_optimizer {
+  std::unique_ptr<symhip::DeviceOptimizer> opt;
+  std::vector<symhip_iteration_stats> iters;
+};
+
+struct symhip_cholesky {
+  symhip::CholSymbolic sym;
+  std::vector<int32_t> colptr, rowind;
+  hipStream_t stream = nullptr;
+  symhip::SupernodalLDLT ldlt;
+  symhip::DBuf<double> vals, b, x;
+  ~symhip_cholesky() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+void copy_stats(const std::vector<symhip_iteration_stats>& v, symhip_iteration_stats* out,
+                int32_t max_iters) {
+  if (!out) return;
+  for (size_t i = 0; i < v.size() && (int32_t)i < max_iters; ++i) out[i] = v[i];
+}
+}  // namespace
+
+extern "C" {
+
+const char* symhip_last_error(void) { return g_last_error.c_str(); }
+
+int symhip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int symhip_problem_create(const symhip_problem_desc* desc, symhip_problem_t* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(out != nullptr, "null output handle");
+    auto p = std::make_unique<symhip_problem>();
+    symhip::build_host_problem(desc, p->host);
+    *out = p.release();
+  });
+}
+
+int symhip_problem_destroy(symhip_problem_t p) {
+  return guarded([&] { delete p; });
+}
+
+int symhip_problem_dims(symhip_problem_t p, int64_t* out) {
+  return guarded([&] {
+    out[0] = p->host.N;
+    out[1] = p->host.M;
+    out[2] = p->host.nnz();
+    out[3] = p->host.storage_total;
+    out[4] = (int64_t)p->host.state_keys.size();
+  });
+}
+
+int symhip_problem_hessian_structure(symhip_problem_t p, int32_t* colptr, int32_t* rowind) {
+  return guarded([&] {
+    std::memcpy(colptr, p->host.colptr.data(), sizeof(int32_t) * p->host.colptr.size());
+    std::memcpy(rowind, p->host.rowind.data(), sizeof(int32_t) * p->host.rowind.size());
+  });
+}
+
+int64_t symhip_problem_col_starts(symhip_problem_t p, int32_t* out) {
+  if (out) std::memcpy(out, p->host.col_starts.data(), sizeof(int32_t) * p->host.col_starts.size());
+  return (int64_t)p->host.col_starts.size();
+}
+
+int symhip_problem_state_keys(symhip_problem_t p, int32_t* key_index, int32_t* tangent_offset) {
+  return guarded([&] {
+    for (size_t i = 0; i < p->host.state_keys.size(); ++i) {
+      key_index[i] = p->host.state_keys[i];
+      tangent_offset[i] = p->host.tan_off[p->host.state_keys[i]];
+    }
+  });
+}
+
+int symhip_linearize(symhip_problem_t p, const double* values, double* residual,
+                     double* hessian_values, double* rhs) {
+  return guarded([&] {
+    p->ensure_device();
+    const auto& h = p->host;
+    p->values.upload(values, h.storage_total, p->stream);
+    p->residual.resize(h.M);
+    p->hval.resize(h.nnz());
+    p->rhs.resize(h.N);
+    p->lin.run(p->values.get(), p->residual.get(), p->hval.get(), p->rhs.get(), p->stream);
+    if (residual) p->residual.download(residual, h.M, p->stream);
+    if (hessian_values) p->hval.download(hessian_values, h.nnz(), p->stream);
+    if (rhs) p->rhs.download(rhs, h.N, p->stream);
+    HIP_OK(hipStreamSynchronize(p->stream));
+  });
+}
+
+int symhip_optimizer_create(const symhip_problem_desc* desc, const symhip_optimizer_params* params,
+                            const symhip_solver_options* options, double epsilon,
+                            symhip_optimizer_t* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(desc && params && out, "null argument");
+    symhip::SolverOptions o;
+    if (options) {
+      o.linear_solver = options->linear_solver;
+      o.ordering = options->ordering;
+      o.precision = options->precision;
+      o.use_pcg = options->use_pcg;
+      o.pcg_max_iters = options->pcg_max_iters;
+      o.pcg_rel_tol = options->pcg_rel_tol;
+    }
+    SYMHIP_CHECK(params->lambda_update_type == SYMHIP_LAMBDA_STATIC ||
+                     params->lambda_update_type == SYMHIP_LAMBDA_DYNAMIC,
+                 "Invalid lambda update type");
+    auto h = std::make_unique<symhip_optimizer>();
+    h->opt = std::make_unique<symhip::DeviceOptimizer>(desc, *params, o, epsilon);
+    *out = h.release();
+  });
+}
+
+int symhip_optimizer_destroy(symhip_optimizer_t o) {
+  return guarded([&] { delete o; });
+}
+
+int symhip_optimizer_optimize(symhip_optimizer_t o, double* values, int32_t num_iterations,
+                              symhip_iteration_stats* iters, int32_t max_iters,
+                              symhip_optimization_stats* stats) {
+  return guarded([&] {
+    symhip_optimization_stats st{};
+    o->opt->optimize(values, num_iterations, false, o->iters, st);
+    copy_stats(o->iters, iters, max_iters);
+    if (stats) *stats = st;
+  });
+}
+
+int symhip_optimizer_best_linearization(symhip_optimizer_t o, double* residual,
+                                        double* hessian_values, double* rhs) {
+  return guarded([&] { o->opt->best_linearization(residual, hessian_values, rhs); });
+}
+
+int symhip_optimizer_update_params(symhip_optimizer_t o, const symhip_optimizer_params* p) {
+  return guarded([&] { o->opt->update_params(*p); });
+}
+
+int symhip_optimizer_set_values(symhip_optimizer_t o, const double* values) {
+  return guarded([&] { o->opt->upload_values(values); });
+}
+
+int symhip_optimizer_optimize_resident(symhip_optimizer_t o, int32_t num_iterations,
+                                       symhip_iteration_stats* iters, int32_t max_iters,
+                                       symhip_optimization_stats* stats) {
+  return guarded([&] {
+    symhip_optimization_stats st{};
+    o->opt->optimize_device(num_iterations, o->iters, st);
+    copy_stats(o->iters, iters, max_iters);
+    if (stats) *stats = st;
+  });
+}
+
+int symhip_optimizer_get_values(symhip_optimizer_t o, double* values) {
+  return guarded([&] { o->opt->download_best_values(values); });
+}
+
+int symhip_optimizer_solver_info(symhip_optimizer_t o, int64_t* out) {
+  return guarded([&] { o->opt->backend().info(out); });
+}
+
+int symhip_cholesky_create(int32_t n, const int32_t* colptr, const int32_t* rowind,
+                           const int32_t* node_ptr, int32_t n_nodes, int32_t ordering,
+                           const int32_t* given_order, symhip_cholesky_t* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(n >= 0 && colptr && rowind && out, "invalid arguments");
+    auto c = std::make_unique<symhip_cholesky>();
+    c->colptr.assign(colptr, colptr + n + 1);
+    c->rowind.assign(rowind, rowind + colptr[n]);
+    std::vector<int32_t> nodes;
+    if (node_ptr) {
+      nodes.assign(node_ptr, node_ptr + n_nodes + 1);
+    } else {
+      nodes.resize(n + 1);
+      for (int32_t i = 0; i <= n; ++i) nodes[i] = i;
+    }
+    std::vector<int32_t> given;
+    symhip::Ordering ord = ordering == SYMHIP_ORDERING_NATURAL ? symhip::Ordering::kNatural
+                                                               : symhip::Ordering::kMinDegree;
+    if (given_order) {
+      given.assign(given_order, given_order + (int32_t)nodes.size() - 1);
+      ord = symhip::Ordering::kGiven;
+    }
+    symhip::chol_symbolic(n, colptr, rowind, nodes, ord, given_order ? &given : nullptr, 64,
+                          c->sym);
+    *out = c.release();
+  });
+}
+
+int symhip_cholesky_destroy(symhip_cholesky_t c) {
+  return guarded([&] { delete c; });
+}
+
+static void ensure_cholesky_device(symhip_cholesky_t c) {
+  if (c->stream) return;
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->ldlt.setup(c->sym, c->colptr.data(), c->rowind.data(), c->stream);
+}
+
+int symhip_cholesky_factorize(symhip_cholesky_t c, const double* values) {
+  return guarded([&] {
+    ensure_cholesky_device(c);
+    c->vals.upload(values, c->rowind.size(), c->stream);
+    c->ldlt.factorize(c->vals.get(), c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int symhip_cholesky_solve(symhip_cholesky_t c, const double* b, double* x, int32_t nrhs) {
+  return guarded([&] {
+    ensure_cholesky_device(c);
+    const int32_t n = c->sym.n;
+    c->b.resize(std::max(1, n));
+    c->x.resize(std::max(1, n));
+    for (int32_t k = 0; k < nrhs; ++k) {
+      c->b.upload(b + (int64_t)k * n, n, c->stream);
+      c->ldlt.solve(c->b.get(), c->x.get(), c->stream);
+      c->x.download(x + (int64_t)k * n, n, c->stream);
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int symhip_cholesky_permutation(symhip_cholesky_t c, int32_t* inv_perm) {
+  return guarded([&] { std::memcpy(inv_perm, c->sym.inv_perm.data(), sizeof(int32_t) * c->sym.n); });
+}
+
+int symhip_cholesky_etree(symhip_cholesky_t c, int32_t* parent, int32_t* counts) {
+  return guarded([&] {
+    std::vector<int32_t> p, k;
+    symhip::scalar_etree_counts(c->sym.n, c->colptr.data(), c->rowind.data(), c->sym.perm, p, k);
+    std::memcpy(parent, p.data(), sizeof(int32_t) * p.size());
+    std::memcpy(counts, k.data(), sizeof(int32_t) * k.size());
+  });
+}
+
+int symhip_cholesky_d(symhip_cholesky_t c, double* d) {
+  return guarded([&] {
+    std::vector<double> panels, dd;
+    c->ldlt.download_factor(panels, dd, c->stream);
+    std::memcpy(d, dd.data(), sizeof(double) * dd.size());
+  });
+}
+
+int symhip_cholesky_info(symhip_cholesky_t c, int64_t* out) {
+  return guarded([&] {
+    out[0] = c->sym.n;
+    out[1] = c->sym.n_snodes;
+    out[2] = c->sym.panel_size();
+    out[3] = c->sym.n_levels;
+    out[4] = (int64_t)c->sym.factor_flops;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Device numeric supernodal LDL^T factorization and solves (see chol_symbolic.h for the layout).
+#pragma once
+
+#include <vector>
+
+#include "./chol_symbolic.h"
+#include "./common.h"
+
+namespace symhip {
+
+// Replaces SparseCholeskySolver::Factorize / Solve (opt/sparse_cholesky/
+// sparse_cholesky_solver.tcc:108-259): A = P^T L D L^T P with unit lower L, computed by dense
+// tile kernels over supernodal panels, level by level in the supernodal elimination tree.
+class SupernodalLDLT {
+ public:
+  // colptr/rowind: lower CSC pattern (host) of the matrices that will be factorized.
+  void setup(const CholSymbolic& sym, const int32_t* colptr, const int32_t* rowind,
+             hipStream_t s);
+  // Assembles A (device values in the CSC order given to setup) into panels and factorizes.
+  void factorize(const double* a_values, hipStream_t s);
+  // Panels already assembled by the caller (e.g. the Schur path writes them directly).
+  void factorize_assembled(hipStream_t s);
+  // x = A^-1 b for device vectors in the original (unpermuted) order; x may alias b.
+  void solve(const double* b, double* x, hipStream_t s);
+
+  double* panels() const { return panel_.get(); }
+  const CholSymbolic& symbolic() const { return sym_; }
+  // Downloads L (dense panels) and D for checking.
+  void download_factor(std::vector<double>& panels, std::vector<double>& d, hipStream_t s) const;
+  int64_t assembly_nnz() const { return (int64_t)asm_map_.size(); }
+
+ private:
+  CholSymbolic sym_;
+  DBuf<double> panel_, D_, y_;
+  DBuf<int64_t> asm_map_, panel_off_;
+  DBuf<int32_t> col_begin_, rows_ptr_, rows_, perm_;
+  DBuf<int32_t> upd_ptr_, upd_target_, upd_seg_begin_, upd_rel_ptr_, upd_rel_, upd_col_off_;
+  // work lists (flattened over levels)
+  std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_;
+  DBuf<int32_t> lvl_snodes_, trsm_items_, upd_items_;
+};
+
+}  // namespace symhip

@@ -1,0 +1,102 @@
+// Device-resident Levenberg-Marquardt optimizer (sym::Optimizer + LevenbergMarquardtSolver).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "./chol_numeric.h"
+#include "./common.h"
+#include "./linearize.h"
+#include "./problem.h"
+
+namespace symhip {
+
+enum LinearSolverKind { kSolverCholesky = 0, kSolverSchur = 1 };
+
+struct SolverOptions {
+  int linear_solver = kSolverCholesky;
+  int ordering = 1;  // 0 natural, 1 minimum degree
+  int precision = 0; // 0 FP64, 1 FP32 Jacobians with FP64 residual/cost (fast form)
+  int use_pcg = 0;   // Schur path: preconditioned CG on the reduced camera system
+  int pcg_max_iters = 100;
+  double pcg_rel_tol = 1e-6;
+};
+
+// A linear system backend owns the linearization storage of the three LM state blocks and the
+// linear solver. The LM logic (optimizer.hip) is independent of the storage format.
+class SystemBackend {
+ public:
+  virtual ~SystemBackend() = default;
+  // Relinearize block b at device values; writes residual(b), the system and err (device scalar).
+  virtual void relinearize(int b, const double* values, double* err_out, hipStream_t s) = 0;
+  // Damp block b's Hessian diagonal (DampHessian, levenberg_marquardt_solver.tcc:22-54); writes
+  // damping (N) and keeps the undamped diagonal for restore().
+  virtual void damp(int b, const double* lambda_dev, const symhip_optimizer_params& p,
+                    bool have_max_diagonal, hipStream_t s) = 0;
+  virtual void restore(int b, hipStream_t s) = 0;
+  // update = -H^-1 rhs for block b (factorize + solve), in the reference state order.
+  virtual void solve(int b, double* update, hipStream_t s) = 0;
+  virtual const double* rhs(int b) const = 0;
+  virtual const double* residual(int b) const = 0;
+  virtual const double* damping() const = 0;
+  // Hessian in reference CSC order (for best-linearization and parity output); may be null.
+  virtual void hessian_csc(int b, double* out_host, hipStream_t s) = 0;
+  virtual int32_t N() const = 0;
+  virtual int32_t M() const = 0;
+  virtual int64_t nnz() const = 0;
+  // out[0]=factorized dim, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=flops
+  virtual void info(int64_t* out5) const = 0;
+};
+
+std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const SolverOptions& o,
+                                                hipStream_t s);
+std::unique_ptr<SystemBackend> make_schur_backend(const HostProblem& p, const SolverOptions& o,
+                                                  hipStream_t s);
+
+struct LmDevState;  // device-side scalars (optimizer.hip)
+
+class DeviceOptimizer {
+ public:
+  DeviceOptimizer(const symhip_problem_desc* d, const symhip_optimizer_params& p,
+                  const SolverOptions& o, double epsilon);
+  ~DeviceOptimizer();
+  // Optimizer::Optimize (opt/optimizer.tcc:70-89 + internal/optimizer_utils.h:36-104).
+  void optimize(double* values_inout, int32_t num_iterations, bool populate_best,
+                std::vector<symhip_iteration_stats>& iters, symhip_optimization_stats& stats);
+  // Same, with Values already on the device (benchmark path: no host transfers in the loop).
+  void optimize_device(int32_t num_iterations, std::vector<symhip_iteration_stats>& iters,
+                       symhip_optimization_stats& stats);
+  void upload_values(const double* values);
+  void download_best_values(double* values);
+  void best_linearization(double* residual, double* hval, double* rhs);
+  void update_params(const symhip_optimizer_params& p) { p_ = p; }
+  const HostProblem& host() const { return host_; }
+  SystemBackend& backend() { return *backend_; }
+  hipStream_t stream() const { return stream_; }
+  double last_iteration_ms() const { return last_iter_ms_; }
+
+ private:
+  bool iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index, int& status,
+               int& reason);
+  void relinearize(int b);
+
+  HostProblem host_;
+  symhip_optimizer_params p_;
+  SolverOptions o_;
+  double epsilon_;
+  hipStream_t stream_ = nullptr;
+  std::unique_ptr<SystemBackend> backend_;
+  DBuf<double> values_[3];
+  DBuf<double> update_, last_update_, partials_;
+  DBuf<int32_t> retract_type_, retract_soff_, retract_toff_;
+  LmDevState* dstate_ = nullptr;  // device
+  LmDevState* hstate_ = nullptr;  // pinned host mirror
+  int init_ = 0, new_ = 1, best_ = 0, free_ = 2;
+  bool valid_[3] = {false, false, false};
+  bool linearized_[3] = {false, false, false};
+  bool have_max_diagonal_ = false, have_last_update_ = false;
+  int iteration_ = -1;
+  double last_iter_ms_ = 0;
+};
+
+}  // namespace symhip

@@ -1,0 +1,482 @@
+// Device-resident Levenberg-Marquardt optimizer.
+//
+// Follows LevenbergMarquardtSolver::Iterate (opt/levenberg_marquardt_solver.tcc:138-343) and the
+// three-block state of LevenbergMarquardtStateBase (opt/internal/levenberg_marquardt_state.h:
+// 33-270). Damping, the linear solve, the retraction, the cost / linear-error reductions and the
+// accept / lambda decision all run on the device; the host reads back one small record per
+// iteration (the iteration statistics) and mirrors the state-block bookkeeping from it.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "./geo.h"
+#include "./optimizer.h"
+#include "./util.h"
+
+namespace symhip {
+
+struct LmDevState {
+  double lambda, nu;
+  double err[3];
+  double lde;
+  double nl2, nu2, dotlu;
+  symhip_iteration_stats rec;
+  int32_t status, reason, accept, set_best;
+};
+
+namespace {
+
+__global__ void lm_reset_kernel(LmDevState* st, double lambda, double nu) {
+  st->lambda = lambda;
+  st->nu = nu;
+  st->status = 0;
+  st->reason = 0;
+  st->accept = 0;
+  st->set_best = 0;
+}
+
+// Record the iteration -1 statistics (levenberg_marquardt_solver.tcc:158-186).
+__global__ void lm_first_record_kernel(LmDevState* st, int init) {
+  st->rec = symhip_iteration_stats{};
+  st->rec.iteration = -1;
+  st->rec.new_error = st->err[init];
+  st->rec.current_lambda = st->lambda;
+  st->status = isfinite(st->err[init]) ? 0 : SYMHIP_STATUS_FAILED;
+  st->reason = isfinite(st->err[init]) ? 0 : SYMHIP_FAILURE_INITIAL_ERROR_NOT_FINITE;
+}
+
+// The accept / reject / lambda logic of Iterate (levenberg_marquardt_solver.tcc:239-340).
+__global__ void lm_decide_kernel(LmDevState* st, int init, int nw, int best, int iteration,
+                                 symhip_optimizer_params p, int have_last_update, double eps) {
+  const double e0 = st->err[init], e1 = st->err[nw];
+  const double rel = (e0 - e1) / (e0 + eps);
+  const double nel = e0 + 0.5 * st->lde;
+  const double gain = (e0 - e1) / (e0 - nel);
+  symhip_iteration_stats rec{};
+  rec.iteration = iteration;
+  rec.current_lambda = st->lambda;
+  rec.new_error = e1;
+  rec.new_error_linear = nel;
+  rec.relative_reduction = rel;
+  int status = 0, reason = 0;
+  if (rel > -p.early_exit_min_reduction / 10 && rel < p.early_exit_min_reduction)
+    status = SYMHIP_STATUS_SUCCESS;
+  else if (e1 < p.early_exit_min_absolute_error)
+    status = SYMHIP_STATUS_SUCCESS;
+  bool accept = rel > 0;
+  double angle = 0;
+  if (p.enable_bold_updates && have_last_update && !accept) {
+    angle = (st->nl2 > 0 && st->nu2 > 0) ? st->dotlu / (sqrt(st->nl2) * sqrt(st->nu2)) : 0.0;
+    accept = ((1 - angle) * (1 - angle) * e1) <= st->err[best];
+  }
+  if (!accept && st->lambda >= p.lambda_upper_bound) {
+    status = SYMHIP_STATUS_FAILED;
+    reason = SYMHIP_FAILURE_LAMBDA_OUT_OF_BOUNDS;
+  }
+  double lambda = st->lambda, nu = st->nu;
+  int set_best = 0;
+  if (!accept) {
+    if (p.lambda_update_type == SYMHIP_LAMBDA_STATIC) {
+      lambda *= p.lambda_up_factor;
+    } else {
+      lambda *= nu;
+      nu *= 2;
+    }
+  } else {
+    if (p.lambda_update_type == SYMHIP_LAMBDA_STATIC) {
+      lambda *= p.lambda_down_factor;
+    } else {
+      lambda *= fmax(1.0 / p.dynamic_lambda_update_gamma,
+                     1.0 - (p.dynamic_lambda_update_beta - 1) *
+                               pow(2 * gain - 1, (double)p.dynamic_lambda_update_p));
+      nu = 2;
+    }
+    set_best = e1 <= st->err[best];
+  }
+  lambda = fmin(fmax(lambda, p.lambda_lower_bound), p.lambda_upper_bound);
+  rec.update_angle_change = angle;
+  rec.update_accepted = accept ? 1 : 0;
+  st->rec = rec;
+  st->lambda = lambda;
+  st->nu = nu;
+  st->status = status;
+  st->reason = reason;
+  st->accept = accept ? 1 : 0;
+  st->set_best = set_best;
+}
+
+// Values::Retract over the optimized keys (opt/values.cc:314-327).
+__global__ void retract_kernel(int32_t n, const int32_t* __restrict__ type,
+                               const int32_t* __restrict__ soff, const int32_t* __restrict__ toff,
+                               const double* __restrict__ update, double* __restrict__ values,
+                               double eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* v = values + soff[i];
+  const double* d = update + toff[i];
+  const int t = type[i];
+  if (t == SYMHIP_TYPE_POSE3) {
+    rot3_retract_inplace(v, d, eps);
+    v[4] += d[3];
+    v[5] += d[4];
+    v[6] += d[5];
+  } else if (t == SYMHIP_TYPE_ROT3) {
+    rot3_retract_inplace(v, d, eps);
+  } else {
+    const int td = t == SYMHIP_TYPE_SCALAR ? 1
+                   : t == SYMHIP_TYPE_VECTOR2 ? 2
+                   : t == SYMHIP_TYPE_VECTOR3 ? 3
+                   : t == SYMHIP_TYPE_VECTOR4 ? 4
+                   : t == SYMHIP_TYPE_VECTOR6 ? 6
+                   : t == SYMHIP_TYPE_MATRIX33 ? 9
+                                                : 36;
+    for (int k = 0; k < td; ++k) v[k] += d[k];
+  }
+}
+
+// sum_i u_i (rhs_i - damping_i u_i)  (Linearization::LinearDeltaError, linearization.h:62-67)
+__global__ void __launch_bounds__(256) lde_partial_kernel(const double* __restrict__ u,
+                                                          const double* __restrict__ rhs,
+                                                          const double* __restrict__ damp,
+                                                          int64_t n, double* __restrict__ part) {
+  __shared__ double ws[4];
+  double acc = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += u[i] * (rhs[i] - damp[i] * u[i]);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+__global__ void sum_into_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double acc = 0;
+    for (int i = 0; i < nb; ++i) acc += part[i];
+    *out = acc;
+  }
+}
+
+// ---- CSC backend kernels ----
+__global__ void damp_csc_kernel(int32_t n, const int32_t* __restrict__ colptr,
+                                double* __restrict__ hval, const double* __restrict__ lambda_p,
+                                int use_diag, int keep_max, int have_max, double dmin, int unit,
+                                double* __restrict__ undamped, double* __restrict__ damping,
+                                double* __restrict__ max_diag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double lambda = *lambda_p;
+  const double u = hval[colptr[j]];
+  undamped[j] = u;
+  double d = 0;
+  if (use_diag) {
+    if (keep_max) {
+      const double md = have_max ? fmax(max_diag[j], u) : fmax(u, dmin);
+      max_diag[j] = md;
+      d = md * lambda;
+    } else {
+      d = fmax(u, dmin) * lambda;
+    }
+  }
+  if (unit) d += lambda;
+  damping[j] = d;
+  hval[colptr[j]] = u + d;
+}
+__global__ void restore_csc_kernel(int32_t n, const int32_t* __restrict__ colptr,
+                                   double* __restrict__ hval, const double* __restrict__ undamped) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) hval[colptr[j]] = undamped[j];
+}
+__global__ void negate_kernel(int32_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = -x[i];
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------------------------
+// CSC backend: reference hessian_lower layout + supernodal LDL^T
+// -------------------------------------------------------------------------------------------
+class CscBackend final : public SystemBackend {
+ public:
+  CscBackend(const HostProblem& p, const SolverOptions& o, hipStream_t s) {
+    lin_.setup(p, s);
+    for (int b = 0; b < 3; ++b) {
+      residual_[b].resize(std::max(1, p.M));
+      hval_[b].resize(std::max<int64_t>(1, p.nnz()));
+      rhs_[b].resize(std::max(1, p.N));
+    }
+    undamped_.resize(std::max(1, p.N));
+    damping_.resize(std::max(1, p.N));
+    max_diag_.resize(std::max(1, p.N));
+    x_.resize(std::max(1, p.N));
+    partials_.resize(kReduceBlocks);
+    colptr_.upload(p.colptr, s);
+    // Symbolic analysis (the reference's AnalyzeSparsityPattern, done once): nodes are the
+    // optimized keys.
+    std::vector<int32_t> node_ptr{0};
+    for (int32_t k : p.state_keys) node_ptr.push_back(node_ptr.back() + tangent_dim(p.types[k]));
+    CholSymbolic sym;
+    chol_symbolic(p.N, p.colptr.data(), p.rowind.data(), node_ptr,
+                  o.ordering == 0 ? Ordering::kNatural : Ordering::kMinDegree, nullptr, 64, sym);
+    chol_.setup(sym, p.colptr.data(), p.rowind.data(), s);
+    N_ = p.N;
+    M_ = p.M;
+    nnz_ = p.nnz();
+  }
+  void relinearize(int b, const double* values, double* err_out, hipStream_t s) override {
+    lin_.run(values, residual_[b].get(), hval_[b].get(), rhs_[b].get(), s);
+    half_squared_norm(residual_[b].get(), M_, partials_.get(), err_out, s);
+  }
+  void damp(int b, const double* lambda_dev, const symhip_optimizer_params& p,
+            bool have_max_diagonal, hipStream_t s) override {
+    hipLaunchKernelGGL(damp_csc_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_,
+                       colptr_.get(), hval_[b].get(), lambda_dev, p.use_diagonal_damping,
+                       p.keep_max_diagonal_damping, have_max_diagonal ? 1 : 0,
+                       p.diagonal_damping_min, p.use_unit_damping, undamped_.get(),
+                       damping_.get(), max_diag_.get());
+  }
+  void restore(int b, hipStream_t s) override {
+    hipLaunchKernelGGL(restore_csc_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_,
+                       colptr_.get(), hval_[b].get(), undamped_.get());
+  }
+  void solve(int b, double* update, hipStream_t s) override {
+    chol_.factorize(hval_[b].get(), s);
+    chol_.solve(rhs_[b].get(), x_.get(), s);
+    hipLaunchKernelGGL(negate_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_, x_.get(),
+                       update);
+  }
+  const double* rhs(int b) const override { return rhs_[b].get(); }
+  const double* residual(int b) const override { return residual_[b].get(); }
+  const double* damping() const override { return damping_.get(); }
+  void hessian_csc(int b, double* out, hipStream_t s) override {
+    hval_[b].download(out, nnz_, s);
+  }
+  int32_t N() const override { return N_; }
+  int32_t M() const override { return M_; }
+  int64_t nnz() const override { return nnz_; }
+  SupernodalLDLT& chol() { return chol_; }
+  void info(int64_t* out) const override {
+    const CholSymbolic& y = chol_.symbolic();
+    out[0] = y.n;
+    out[1] = y.n_snodes;
+    out[2] = y.panel_size();
+    out[3] = y.n_levels;
+    out[4] = (int64_t)y.factor_flops;
+  }
+
+ private:
+  CscLinearizer lin_;
+  SupernodalLDLT chol_;
+  DBuf<double> residual_[3], hval_[3], rhs_[3];
+  DBuf<double> undamped_, damping_, max_diag_, x_, partials_;
+  DBuf<int32_t> colptr_;
+  int32_t N_ = 0, M_ = 0;
+  int64_t nnz_ = 0;
+};
+
+std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const SolverOptions& o,
+                                                hipStream_t s) {
+  return std::make_unique<CscBackend>(p, o, s);
+}
+
+// -------------------------------------------------------------------------------------------
+// DeviceOptimizer
+// -------------------------------------------------------------------------------------------
+DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_optimizer_params& p,
+                                 const SolverOptions& o, double epsilon)
+    : p_(p), o_(o), epsilon_(epsilon) {
+  build_host_problem(d, host_);
+  HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  backend_ = o.linear_solver == kSolverSchur ? make_schur_backend(host_, o, stream_)
+                                             : make_csc_backend(host_, o, stream_);
+  for (auto& v : values_) v.resize(std::max<int64_t>(1, host_.storage_total));
+  update_.resize(std::max(1, host_.N));
+  last_update_.resize(std::max(1, host_.N));
+  partials_.resize(kReduceBlocks);
+  std::vector<int32_t> rt, rs, rto;
+  for (int32_t k : host_.state_keys) {
+    rt.push_back(host_.types[k]);
+    rs.push_back(host_.storage_offset[k]);
+    rto.push_back(host_.tan_off[k]);
+  }
+  retract_type_.upload(rt, stream_);
+  retract_soff_.upload(rs, stream_);
+  retract_toff_.upload(rto, stream_);
+  HIP_OK(hipMalloc(&dstate_, sizeof(LmDevState)));
+  HIP_OK(hipHostMalloc(&hstate_, sizeof(LmDevState), hipHostMallocDefault));
+  std::memset(hstate_, 0, sizeof(LmDevState));
+  values_[0].upload(host_.values, stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+DeviceOptimizer::~DeviceOptimizer() {
+  if (dstate_) (void)hipFree(dstate_);
+  if (hstate_) (void)hipHostFree(hstate_);
+  backend_.reset();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceOptimizer::upload_values(const double* values) {
+  values_[new_].upload(values, host_.storage_total, stream_);
+}
+
+void DeviceOptimizer::download_best_values(double* values) {
+  values_[best_].download(values, host_.storage_total, stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void DeviceOptimizer::best_linearization(double* residual, double* hval, double* rhs) {
+  if (residual) HIP_OK(hipMemcpyAsync(residual, backend_->residual(best_), sizeof(double) * host_.M,
+                                      hipMemcpyDeviceToHost, stream_));
+  if (rhs) HIP_OK(hipMemcpyAsync(rhs, backend_->rhs(best_), sizeof(double) * host_.N,
+                                 hipMemcpyDeviceToHost, stream_));
+  if (hval) backend_->hessian_csc(best_, hval, stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void DeviceOptimizer::relinearize(int b) {
+  backend_->relinearize(b, values_[b].get(), &dstate_->err[b], stream_);
+  linearized_[b] = true;
+}
+
+void DeviceOptimizer::optimize(double* values_inout, int32_t num_iterations, bool populate_best,
+                               std::vector<symhip_iteration_stats>& iters,
+                               symhip_optimization_stats& stats) {
+  // Reset: the input Values go into the New block (LevenbergMarquardtState::ResetValues).
+  init_ = 0;
+  new_ = 1;
+  best_ = 0;
+  free_ = 2;
+  values_[new_].upload(values_inout, host_.storage_total, stream_);
+  optimize_device(num_iterations, iters, stats);
+  download_best_values(values_inout);
+  (void)populate_best;
+}
+
+void DeviceOptimizer::optimize_device(int32_t num_iterations,
+                                      std::vector<symhip_iteration_stats>& iters,
+                                      symhip_optimization_stats& stats) {
+  const int32_t n = num_iterations < 0 ? p_.iterations : num_iterations;
+  SYMHIP_CHECK(n > 0, "num_iterations must be positive");
+  // Reset (levenberg_marquardt_solver.h:163-183); values live in values_[new_].
+  if (new_ != 1) {
+    HIP_OK(hipMemcpyAsync(values_[1].get(), values_[new_].get(),
+                          sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice, stream_));
+  }
+  init_ = 0;
+  new_ = 1;
+  best_ = 0;
+  free_ = 2;
+  iteration_ = -1;
+  have_max_diagonal_ = false;
+  have_last_update_ = false;
+  for (int b = 0; b < 3; ++b) linearized_[b] = false;
+  valid_[new_] = true;
+  valid_[init_] = valid_[free_] = false;
+  hipLaunchKernelGGL(lm_reset_kernel, dim3(1), dim3(1), 0, stream_, dstate_, p_.initial_lambda,
+                     p_.dynamic_lambda_update_beta);
+  iters.clear();
+  int32_t best_index = 0;
+  int status = 0, reason = 0;
+  int i;
+  for (i = 0; i < n; ++i) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool done = iterate(iters, best_index, status, reason);
+    last_iter_ms_ =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (done) break;
+  }
+  if (i == n) {
+    status = SYMHIP_STATUS_HIT_ITERATION_LIMIT;
+    reason = 0;
+  }
+  stats.num_iterations = (int32_t)iters.size();
+  stats.best_index = best_index;
+  stats.status = status;
+  stats.failure_reason = reason;
+}
+
+bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index,
+                              int& status, int& reason) {
+  std::swap(init_, new_);  // state_.Step()
+  iteration_++;
+  if (!linearized_[init_]) {
+    relinearize(init_);
+    // SetBestToInit
+    if (best_ != init_) {
+      if (best_ != new_) free_ = best_;
+      best_ = init_;
+    }
+  }
+  if (iteration_ == 0) {
+    hipLaunchKernelGGL(lm_first_record_kernel, dim3(1), dim3(1), 0, stream_, dstate_, init_);
+    HIP_OK(hipMemcpyAsync(hstate_, dstate_, sizeof(LmDevState), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    iters.push_back(hstate_->rec);
+    if (hstate_->status == SYMHIP_STATUS_FAILED) {
+      status = hstate_->status;
+      reason = hstate_->reason;
+      return true;
+    }
+  }
+  const int32_t N = host_.N;
+  backend_->damp(init_, &dstate_->lambda, p_, have_max_diagonal_, stream_);
+  if (p_.use_diagonal_damping && p_.keep_max_diagonal_damping) have_max_diagonal_ = true;
+  backend_->solve(init_, update_.get(), stream_);
+  backend_->restore(init_, stream_);
+  // UpdateNewFromInit: copy Init values into New, then retract the optimized keys.
+  HIP_OK(hipMemcpyAsync(values_[new_].get(), values_[init_].get(),
+                        sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice, stream_));
+  valid_[new_] = true;
+  const int32_t nk = (int32_t)host_.state_keys.size();
+  hipLaunchKernelGGL(retract_kernel, dim3(grid_for(nk, 256)), dim3(256), 0, stream_, nk,
+                     retract_type_.get(), retract_soff_.get(), retract_toff_.get(), update_.get(),
+                     values_[new_].get(), epsilon_);
+  relinearize(new_);
+  {
+    const int nb = std::max(1, std::min(kReduceBlocks, grid_for(N, 256)));
+    hipLaunchKernelGGL(lde_partial_kernel, dim3(nb), dim3(256), 0, stream_, update_.get(),
+                       backend_->rhs(init_), backend_->damping(), (int64_t)N, partials_.get());
+    hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(1), 0, stream_, partials_.get(), nb,
+                       &dstate_->lde);
+  }
+  if (p_.enable_bold_updates && have_last_update_) {
+    dot(last_update_.get(), last_update_.get(), N, partials_.get(), &dstate_->nl2, stream_);
+    dot(update_.get(), update_.get(), N, partials_.get(), &dstate_->nu2, stream_);
+    dot(last_update_.get(), update_.get(), N, partials_.get(), &dstate_->dotlu, stream_);
+  }
+  hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(1), 0, stream_, dstate_, init_, new_, best_,
+                     iteration_, p_, have_last_update_ ? 1 : 0, epsilon_);
+  HIP_OK(hipMemcpyAsync(hstate_, dstate_, sizeof(LmDevState), hipMemcpyDeviceToHost, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+  HIP_OK(hipGetLastError());
+  iters.push_back(hstate_->rec);
+  if (!hstate_->accept) {
+    std::swap(new_, init_);  // SwapNewAndInit
+  } else {
+    have_last_update_ = true;
+    HIP_OK(hipMemcpyAsync(last_update_.get(), update_.get(), sizeof(double) * N,
+                          hipMemcpyDeviceToDevice, stream_));
+    if (hstate_->set_best) {
+      // SetBestToNew
+      if (best_ != new_) {
+        if (best_ != init_) free_ = best_;
+        best_ = new_;
+      }
+      best_index = (int32_t)iters.size() - 1;
+    }
+    // SetInitToNotBest
+    if (best_ == init_) {
+      init_ = free_;
+      free_ = best_;
+    }
+  }
+  status = hstate_->status;
+  reason = hstate_->reason;
+  return status != 0;
+}
+
+}  // namespace symhip
