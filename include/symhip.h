@@ -98,9 +98,34 @@ int symhip_optimizer_optimize_resident(symhip_optimizer_t opt, int32_t num_itera
                                        symhip_iteration_stats* iters, int32_t max_iters,
                                        symhip_optimization_stats* stats);
 int symhip_optimizer_get_values(symhip_optimizer_t opt, double* values);
+/* Host wall-clock time (ms since the start of the last optimize call) at which each iteration
+   record of that call became available. Returns the number of entries written. */
+int32_t symhip_optimizer_iteration_times(symhip_optimizer_t opt, double* ms, int32_t max);
+/* Device phases timed with HIP events when timing is enabled (one phase = the kernels of one
+   step of Iterate; the Schur phases are single kernels). */
+enum symhip_phase {
+  SYMHIP_PHASE_LINEARIZE = 0,    /* relinearization of the new state block (+ cost reduction) */
+  SYMHIP_PHASE_DAMP = 1,         /* DampHessian */
+  SYMHIP_PHASE_POINT_ELIM = 2,   /* Schur: landmark block factorization, Z = W L^-T */
+  SYMHIP_PHASE_RCS_ASSEMBLE = 3, /* Schur: reduced camera system blocks into the LDL^T panels */
+  SYMHIP_PHASE_REDUCED_RHS = 4,  /* Schur: reduced right-hand side */
+  SYMHIP_PHASE_FACTOR = 5,       /* supernodal LDL^T factorization (incl. assembly) */
+  SYMHIP_PHASE_SOLVE = 6,        /* triangular solves */
+  SYMHIP_PHASE_BACK_SUB = 7,     /* Schur: landmark back-substitution */
+  SYMHIP_PHASE_RETRACT = 8,      /* UpdateNewFromInit: copy + retraction */
+  SYMHIP_PHASE_DECIDE = 9,       /* linear error reduction + accept / lambda decision */
+  SYMHIP_PHASE_LIN_POINTS = 10,  /* Schur linearization: landmark-major kernel alone */
+  SYMHIP_PHASE_LIN_CAMS = 11,    /* Schur linearization: camera-major kernel alone */
+  SYMHIP_NUM_PHASES = 12,
+};
+int symhip_optimizer_enable_timing(symhip_optimizer_t opt, int32_t enable);
+/* Accumulated milliseconds and occurrence counts per phase since timing was enabled. */
+int symhip_optimizer_phase_times(symhip_optimizer_t opt, double* total_ms, int64_t* counts);
+
 /* Symbolic facts of the linear solver: out[0]=factorized dim, out[1]=supernodes,
-   out[2]=panel doubles, out[3]=levels, out[4]=factor flops (rounded). */
-int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out5);
+   out[2]=panel doubles, out[3]=levels, out[4]=factor flops (rounded), out[5]=reduced-camera-
+   system blocks (Schur path, else 0), out[6]=landmark observation pairs, out[7]=observations. */
+int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out8);
 
 /* ---------------------------------------------------------------------------------------------
  * Linear solver adapter: SparseCholeskySolver (opt/sparse_cholesky/sparse_cholesky_solver.h:

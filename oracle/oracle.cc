@@ -14,6 +14,7 @@
 #include "./oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1108,6 +1109,20 @@ void oracle_default_params(symhip_optimizer_params* p) {
   p->enable_bold_updates = 0;
 }
 
+static double* g_iter_seconds = nullptr;
+
+int oracle_optimize_timed(oracle_problem* op, const symhip_optimizer_params* params, int solver,
+                          const int32_t* ordering, int32_t c_dim, double epsilon,
+                          int32_t num_iterations, double* values_inout,
+                          symhip_iteration_stats* iters, int32_t max_iters,
+                          symhip_optimization_stats* out_stats, double* iter_seconds) {
+  g_iter_seconds = iter_seconds;
+  const int rc = oracle_optimize(op, params, solver, ordering, c_dim, epsilon, num_iterations,
+                                 values_inout, iters, max_iters, out_stats, nullptr);
+  g_iter_seconds = nullptr;
+  return rc;
+}
+
 int oracle_optimize(oracle_problem* op, const symhip_optimizer_params* params, int solver,
                     const int32_t* ordering, int32_t c_dim, double epsilon, int32_t num_iterations,
                     double* values_inout, symhip_iteration_stats* iters, int32_t max_iters,
@@ -1136,7 +1151,11 @@ int oracle_optimize(oracle_problem* op, const symhip_optimizer_params* params, i
   int status = SYMHIP_STATUS_INVALID, reason = 0;
   int i;
   for (i = 0; i < n; ++i) {
+    const auto t0 = std::chrono::steady_clock::now();
     auto st = lm.Iterate(stats, best_index);
+    if (g_iter_seconds)
+      g_iter_seconds[i] =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (st) {
       status = st->first;
       reason = st->second;

@@ -77,6 +77,14 @@ int oracle_optimize(oracle_problem* p, const symhip_optimizer_params* params, in
                     double* values_inout, symhip_iteration_stats* iters, int32_t max_iters,
                     symhip_optimization_stats* out_stats, double* updates_out);
 
+/* As oracle_optimize, also writing the wall time of each Iterate() call (seconds) into
+   iter_seconds (capacity num_iterations). Iteration 0 includes the one-time symbolic analysis. */
+int oracle_optimize_timed(oracle_problem* p, const symhip_optimizer_params* params, int solver,
+                          const int32_t* ordering, int32_t c_dim, double epsilon,
+                          int32_t num_iterations, double* values_inout,
+                          symhip_iteration_stats* iters, int32_t max_iters,
+                          symhip_optimization_stats* out_stats, double* iter_seconds);
+
 /* Default parameters (optimizer.cc:8-56). */
 void oracle_default_params(symhip_optimizer_params* p);
 

@@ -326,6 +326,9 @@ class SolverOptions(C.Structure):
 
 
 SOLVER_CHOLESKY, SOLVER_SCHUR = 0, 1
+PHASE_NAMES = ["linearize", "damp", "point_elim", "rcs_assemble", "reduced_rhs", "factor",
+               "solve", "back_sub", "retract", "decide", "lin_points", "lin_cams"]
+NUM_PHASES = len(PHASE_NAMES)
 ORDERING_NATURAL, ORDERING_MIN_DEGREE = 0, 1
 PRECISION_F64, PRECISION_F32_JAC = 0, 1
 
@@ -343,6 +346,9 @@ _EXTRA_SIGNATURES: list = [
                                             C.POINTER(OptimizationStats)], C.c_int),
     ("symhip_optimizer_get_values", [_vp, F64P], C.c_int),
     ("symhip_optimizer_solver_info", [_vp, I64P], C.c_int),
+    ("symhip_optimizer_iteration_times", [_vp, F64P, C.c_int32], C.c_int32),
+    ("symhip_optimizer_enable_timing", [_vp, C.c_int32], C.c_int),
+    ("symhip_optimizer_phase_times", [_vp, F64P, I64P], C.c_int),
     ("symhip_cholesky_create", [C.c_int32, I32P, I32P, I32P, C.c_int32, C.c_int32, I32P,
                                 C.POINTER(_vp)], C.c_int),
     ("symhip_cholesky_destroy", [_vp], C.c_int),
@@ -465,11 +471,27 @@ class HipOptimizer:
             self.h, ptr(r, C.c_double), ptr(h, C.c_double), ptr(b, C.c_double)))
         return dict(residual=r, hessian_values=h, rhs=b)
 
+    def iteration_times(self):
+        out = np.zeros(4096)
+        n = hip_lib().symhip_optimizer_iteration_times(self.h, ptr(out, C.c_double), 4096)
+        return out[:n]
+
+    def enable_timing(self, on=True):
+        check(hip_lib().symhip_optimizer_enable_timing(self.h, int(on)))
+
+    def phase_times(self):
+        ms = np.zeros(NUM_PHASES)
+        cnt = np.zeros(NUM_PHASES, np.int64)
+        check(hip_lib().symhip_optimizer_phase_times(self.h, ptr(ms, C.c_double),
+                                                     ptr(cnt, C.c_int64)))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(PHASE_NAMES)}
+
     def solver_info(self):
-        out = np.zeros(5, np.int64)
+        out = np.zeros(8, np.int64)
         check(hip_lib().symhip_optimizer_solver_info(self.h, ptr(out, C.c_int64)))
         return dict(n=int(out[0]), supernodes=int(out[1]), panel=int(out[2]),
-                    levels=int(out[3]), flops=int(out[4]))
+                    levels=int(out[3]), flops=int(out[4]), blocks=int(out[5]),
+                    pairs=int(out[6]), obs=int(out[7]))
 
 
 class HipCholesky:

@@ -68,6 +68,16 @@ def build_hip(jobs: int = 8):
     os.replace(tmp, OUT / "libsymhip.so")
 
 
+def build_cpp_tests():
+    """C++ programs written against the sym:: host surface (include/sym/), run by the GPU tests."""
+    out = REPO / "tests" / "cpp" / "_build"
+    out.mkdir(parents=True, exist_ok=True)
+    for src in (REPO / "tests" / "cpp").glob("*.cc"):
+        _run(["g++", "-O2", "-std=c++17", "-I", str(REPO / "include"), str(src), "-o",
+              str(out / src.stem), "-L", str(OUT), "-lsymhip",
+              "-Wl,-rpath,$ORIGIN/../../../symforce_amd/_build"])
+
+
 def build_oracle():
     _run(["make", "-s", "-C", str(REPO / "oracle")])
 
@@ -79,11 +89,13 @@ def build_all(only: str | None = None):
         build_oracle()
     if only in (None, "hip"):
         build_hip()
+    if only in (None, "cpp"):
+        build_cpp_tests()
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["hip", "problems", "oracle"], default=None)
+    ap.add_argument("--only", choices=["hip", "problems", "oracle", "cpp"], default=None)
     a = ap.parse_args()
     build_all(a.only)
     print("built", sorted(p.name for p in OUT.glob("*.so")))

@@ -209,6 +209,26 @@ int symhip_optimizer_get_values(symhip_optimizer_t o, double* values) {
   return guarded([&] { o->opt->download_best_values(values); });
 }
 
+int32_t symhip_optimizer_iteration_times(symhip_optimizer_t o, double* ms, int32_t max) {
+  const auto& t = o->opt->iteration_times();
+  int32_t n = 0;
+  for (; n < (int32_t)t.size() && n < max; ++n) ms[n] = t[n];
+  return n;
+}
+
+int symhip_optimizer_enable_timing(symhip_optimizer_t o, int32_t enable) {
+  return guarded([&] { o->opt->timer().enable(enable != 0); });
+}
+
+int symhip_optimizer_phase_times(symhip_optimizer_t o, double* total_ms, int64_t* counts) {
+  return guarded([&] {
+    for (int i = 0; i < SYMHIP_NUM_PHASES; ++i) {
+      if (total_ms) total_ms[i] = o->opt->timer().total_ms(i);
+      if (counts) counts[i] = o->opt->timer().count(i);
+    }
+  });
+}
+
 int symhip_optimizer_solver_info(symhip_optimizer_t o, int64_t* out) {
   return guarded([&] { o->opt->backend().info(out); });
 }

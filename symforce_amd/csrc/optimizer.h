@@ -8,6 +8,7 @@
 #include "./common.h"
 #include "./linearize.h"
 #include "./problem.h"
+#include "./timer.h"
 
 namespace symhip {
 
@@ -44,8 +45,13 @@ class SystemBackend {
   virtual int32_t N() const = 0;
   virtual int32_t M() const = 0;
   virtual int64_t nnz() const = 0;
-  // out[0]=factorized dim, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=flops
-  virtual void info(int64_t* out5) const = 0;
+  // out[0]=factorized dim, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=flops,
+  // out[5]=RCS blocks, out[6]=observation pairs, out[7]=observations
+  virtual void info(int64_t* out8) const = 0;
+  void set_timer(PhaseTimer* t) { timer_ = t; }
+
+ protected:
+  PhaseTimer* timer_ = nullptr;
 };
 
 std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const SolverOptions& o,
@@ -74,6 +80,8 @@ class DeviceOptimizer {
   SystemBackend& backend() { return *backend_; }
   hipStream_t stream() const { return stream_; }
   double last_iteration_ms() const { return last_iter_ms_; }
+  PhaseTimer& timer() { return timer_; }
+  const std::vector<double>& iteration_times() const { return iter_times_; }
 
  private:
   bool iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index, int& status,
@@ -97,6 +105,10 @@ class DeviceOptimizer {
   bool have_max_diagonal_ = false, have_last_update_ = false;
   int iteration_ = -1;
   double last_iter_ms_ = 0;
+  std::vector<double> iter_times_;
+  DBuf<double> start_values_;
+  bool have_start_ = false;
+  PhaseTimer timer_;
 };
 
 }  // namespace symhip

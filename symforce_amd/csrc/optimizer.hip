@@ -243,7 +243,11 @@ class CscBackend final : public SystemBackend {
                        colptr_.get(), hval_[b].get(), undamped_.get());
   }
   void solve(int b, double* update, hipStream_t s) override {
-    chol_.factorize(hval_[b].get(), s);
+    {
+      PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
+      chol_.factorize(hval_[b].get(), s);
+    }
+    PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
     chol_.solve(rhs_[b].get(), x_.get(), s);
     hipLaunchKernelGGL(negate_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_, x_.get(),
                        update);
@@ -265,6 +269,7 @@ class CscBackend final : public SystemBackend {
     out[2] = y.panel_size();
     out[3] = y.n_levels;
     out[4] = (int64_t)y.factor_flops;
+    out[5] = out[6] = out[7] = 0;
   }
 
  private:
@@ -292,6 +297,7 @@ DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_opti
   HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   backend_ = o.linear_solver == kSolverSchur ? make_schur_backend(host_, o, stream_)
                                              : make_csc_backend(host_, o, stream_);
+  backend_->set_timer(&timer_);
   for (auto& v : values_) v.resize(std::max<int64_t>(1, host_.storage_total));
   update_.resize(std::max(1, host_.N));
   last_update_.resize(std::max(1, host_.N));
@@ -308,8 +314,7 @@ DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_opti
   HIP_OK(hipMalloc(&dstate_, sizeof(LmDevState)));
   HIP_OK(hipHostMalloc(&hstate_, sizeof(LmDevState), hipHostMallocDefault));
   std::memset(hstate_, 0, sizeof(LmDevState));
-  values_[0].upload(host_.values, stream_);
-  HIP_OK(hipStreamSynchronize(stream_));
+  upload_values(host_.values.data());
 }
 
 DeviceOptimizer::~DeviceOptimizer() {
@@ -320,7 +325,9 @@ DeviceOptimizer::~DeviceOptimizer() {
 }
 
 void DeviceOptimizer::upload_values(const double* values) {
-  values_[new_].upload(values, host_.storage_total, stream_);
+  start_values_.upload(values, host_.storage_total, stream_);
+  have_start_ = true;
+  HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void DeviceOptimizer::download_best_values(double* values) {
@@ -338,6 +345,7 @@ void DeviceOptimizer::best_linearization(double* residual, double* hval, double*
 }
 
 void DeviceOptimizer::relinearize(int b) {
+  PhaseScope ps(&timer_, SYMHIP_PHASE_LINEARIZE, stream_);
   backend_->relinearize(b, values_[b].get(), &dstate_->err[b], stream_);
   linearized_[b] = true;
 }
@@ -350,7 +358,7 @@ void DeviceOptimizer::optimize(double* values_inout, int32_t num_iterations, boo
   new_ = 1;
   best_ = 0;
   free_ = 2;
-  values_[new_].upload(values_inout, host_.storage_total, stream_);
+  upload_values(values_inout);
   optimize_device(num_iterations, iters, stats);
   download_best_values(values_inout);
   (void)populate_best;
@@ -361,11 +369,10 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
                                       symhip_optimization_stats& stats) {
   const int32_t n = num_iterations < 0 ? p_.iterations : num_iterations;
   SYMHIP_CHECK(n > 0, "num_iterations must be positive");
-  // Reset (levenberg_marquardt_solver.h:163-183); values live in values_[new_].
-  if (new_ != 1) {
-    HIP_OK(hipMemcpyAsync(values_[1].get(), values_[new_].get(),
-                          sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice, stream_));
-  }
+  // Reset (levenberg_marquardt_solver.h:163-183): the start Values go into the New block.
+  SYMHIP_CHECK(have_start_, "optimize called before the Values were set");
+  HIP_OK(hipMemcpyAsync(values_[1].get(), start_values_.get(), sizeof(double) * host_.storage_total,
+                        hipMemcpyDeviceToDevice, stream_));
   init_ = 0;
   new_ = 1;
   best_ = 0;
@@ -379,14 +386,19 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
   hipLaunchKernelGGL(lm_reset_kernel, dim3(1), dim3(1), 0, stream_, dstate_, p_.initial_lambda,
                      p_.dynamic_lambda_update_beta);
   iters.clear();
+  iter_times_.clear();
   int32_t best_index = 0;
   int status = 0, reason = 0;
   int i;
+  const auto t_start = std::chrono::steady_clock::now();
   for (i = 0; i < n; ++i) {
     const auto t0 = std::chrono::steady_clock::now();
+    const size_t before = iters.size();
     const bool done = iterate(iters, best_index, status, reason);
-    last_iter_ms_ =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const auto t1 = std::chrono::steady_clock::now();
+    last_iter_ms_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    const double since = std::chrono::duration<double, std::milli>(t1 - t_start).count();
+    for (size_t k = before; k < iters.size(); ++k) iter_times_.push_back(since);
     if (done) break;
   }
   if (i == n) {
@@ -423,19 +435,27 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
     }
   }
   const int32_t N = host_.N;
-  backend_->damp(init_, &dstate_->lambda, p_, have_max_diagonal_, stream_);
+  {
+    PhaseScope ps(&timer_, SYMHIP_PHASE_DAMP, stream_);
+    backend_->damp(init_, &dstate_->lambda, p_, have_max_diagonal_, stream_);
+  }
   if (p_.use_diagonal_damping && p_.keep_max_diagonal_damping) have_max_diagonal_ = true;
   backend_->solve(init_, update_.get(), stream_);
   backend_->restore(init_, stream_);
   // UpdateNewFromInit: copy Init values into New, then retract the optimized keys.
-  HIP_OK(hipMemcpyAsync(values_[new_].get(), values_[init_].get(),
-                        sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice, stream_));
-  valid_[new_] = true;
-  const int32_t nk = (int32_t)host_.state_keys.size();
-  hipLaunchKernelGGL(retract_kernel, dim3(grid_for(nk, 256)), dim3(256), 0, stream_, nk,
-                     retract_type_.get(), retract_soff_.get(), retract_toff_.get(), update_.get(),
-                     values_[new_].get(), epsilon_);
+  {
+    PhaseScope ps(&timer_, SYMHIP_PHASE_RETRACT, stream_);
+    HIP_OK(hipMemcpyAsync(values_[new_].get(), values_[init_].get(),
+                          sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice,
+                          stream_));
+    valid_[new_] = true;
+    const int32_t nk = (int32_t)host_.state_keys.size();
+    hipLaunchKernelGGL(retract_kernel, dim3(grid_for(nk, 256)), dim3(256), 0, stream_, nk,
+                       retract_type_.get(), retract_soff_.get(), retract_toff_.get(),
+                       update_.get(), values_[new_].get(), epsilon_);
+  }
   relinearize(new_);
+  const int decide_ev = timer_.begin(SYMHIP_PHASE_DECIDE, stream_);
   {
     const int nb = std::max(1, std::min(kReduceBlocks, grid_for(N, 256)));
     hipLaunchKernelGGL(lde_partial_kernel, dim3(nb), dim3(256), 0, stream_, update_.get(),
@@ -450,8 +470,10 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   }
   hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(1), 0, stream_, dstate_, init_, new_, best_,
                      iteration_, p_, have_last_update_ ? 1 : 0, epsilon_);
+  timer_.end(decide_ev, stream_);
   HIP_OK(hipMemcpyAsync(hstate_, dstate_, sizeof(LmDevState), hipMemcpyDeviceToHost, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
+  timer_.collect();
   HIP_OK(hipGetLastError());
   iters.push_back(hstate_->rec);
   if (!hstate_->accept) {

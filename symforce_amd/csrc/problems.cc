@@ -380,8 +380,9 @@ symhip_generated* symhip_gen_ba_snavely(int32_t n_cams, int32_t n_points, uint32
     for (int i = s; i < s + t; ++i) {
       double q[3];
       pact(cam_T_world[i], pts[j].data(), q);
-      if (q[2] > -1.0) continue;  // behind or too close: not observed
+      if (q[2] > -3.0) continue;  // behind or too close: not observed
       const double px = -q[0] / q[2], py = -q[1] / q[2];
+      if (std::fabs(px) > 1.2 || std::fabs(py) > 1.2) continue;  // outside the field of view
       const double r2 = px * px + py * py;
       const double rad = 1.0 + intr[i][1] * r2 + intr[i][2] * r2 * r2;
       obs.push_back({i, j, intr[i][0] * rad * px + nrm(gen), intr[i][0] * rad * py + nrm(gen)});
@@ -389,10 +390,13 @@ symhip_generated* symhip_gen_ba_snavely(int32_t n_cams, int32_t n_points, uint32
   }
   std::vector<int32_t> kc(n_cams), ki(n_cams), kp(n_points);
   for (int i = 0; i < n_cams; ++i) {
+    // perturb the camera's pose in the world (orientation about its own center, position), then
+    // store the inverse as cam_T_world
     double d[6];
     for (int k = 0; k < 3; ++k) d[k] = 0.05 * nrm(gen);
     for (int k = 3; k < 6; ++k) d[k] = 0.1 * nrm(gen);
-    kc[i] = g->add_pose('c', kInvalid, i, i == 0 ? cam_T_world[0] : pretract(cam_T_world[i], d, 0.0));
+    kc[i] = g->add_pose('c', kInvalid, i,
+                        i == 0 ? cam_T_world[0] : pinv(pretract(world_T_cam[i], d, 0.0)));
   }
   for (int i = 0; i < n_cams; ++i)
     ki[i] = g->add_key('i', kInvalid, i, SYMHIP_TYPE_VECTOR3, intr[i].data(), 3);

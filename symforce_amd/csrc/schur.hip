@@ -1,11 +1,930 @@
-// Schur-complement backend (filled in below).
+// Schur-complement backend: landmark-block elimination + reduced-camera-system (RCS) assembly.
+//
+// Replaces SparseSchurSolver (opt/sparse_schur_solver.tcc:16-163) behind the LM loop for problems
+// whose trailing optimized keys are 3-dof landmarks, each touched by factors that otherwise touch
+// one "camera node" (the B keys that always appear together, e.g. Pose3 + intrinsics). With
+// V_p the 3x3 landmark block, W_o = J_node^T J_point per observation and L_p L_p^T = V_p:
+//   Z_o = W_o L_p^-T,  h_p = L_p^-1 g_p,
+//   S(a,b) = B(a,b) - sum_{p in a,b} Z_{p,a} Z_{p,b}^T,  v_a = g_a - sum_{o in a} Z_o h_p,
+//   S y = v,  z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)).
+// Layout in HBM: observations are stored point-major (points sorted by first camera), with a
+// camera-major index for the camera reductions; S is never materialized separately, each S block
+// is reduced by one wavefront over its precomputed list of observation pairs and written straight
+// into the supernodal LDL^T panels.
+#include <algorithm>
+#include <numeric>
+
+#include "./chol_numeric.h"
+#include "./factor_traits.h"
 #include "./optimizer.h"
+#include "./util.h"
 
 namespace symhip {
 
-std::unique_ptr<SystemBackend> make_schur_backend(const HostProblem&, const SolverOptions&,
-                                                  hipStream_t) {
-  throw Error("symhip: Schur backend not available yet");
+namespace {
+
+struct ObsView {
+  int32_t n_obs;
+  const int32_t* arg_off;   // [NA][n_obs]
+  const int32_t* res_off;   // [n_obs]
+  const int32_t* node;      // [n_obs]
+};
+
+// Per point (thread): evaluate its observations, write residuals, W_o (ND x 3 row-major) and the
+// point block V_p (lower 6: 00 10 11 20 21 22) and g_p (3).
+template <int FTC, int ND>
+__global__ void __launch_bounds__(128) lin_points_kernel(int32_t n_points,
+                                                         const int32_t* __restrict__ pt_obs_ptr,
+                                                         ObsView ov, const double* __restrict__ values,
+                                                         double* __restrict__ residual,
+                                                         double* __restrict__ W,
+                                                         double* __restrict__ V,
+                                                         double* __restrict__ gp) {
+  using F = FT<FTC>;
+  static_assert(F::T == ND + 3, "landmark factor layout");
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_points) return;
+  double v[6] = {}, g[3] = {};
+  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
+    const double* a[F::NA];
+#pragma unroll
+    for (int k = 0; k < F::NA; ++k) a[k] = values + ov.arg_off[k * ov.n_obs + o];
+    double r[F::D], J[F::D][F::T];
+    F::template eval<double>(a, r, J);
+    const int ro = ov.res_off[o];
+#pragma unroll
+    for (int i = 0; i < F::D; ++i) residual[ro + i] = r[i];
+    double* Wo = W + (int64_t)o * ND * 3;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double acc = 0;
+#pragma unroll
+        for (int q = 0; q < F::D; ++q) acc += J[q][i] * J[q][ND + k];
+        Wo[i * 3 + k] = acc;
+      }
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double acc = 0;
+#pragma unroll
+      for (int q = 0; q < F::D; ++q) acc += J[q][ND + i] * r[q];
+      g[i] += acc;
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double h = 0;
+#pragma unroll
+        for (int q = 0; q < F::D; ++q) h += J[q][ND + i] * J[q][ND + j];
+        v[idx++] += h;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) V[(int64_t)p * 6 + i] = v[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) gp[(int64_t)p * 3 + i] = g[i];
+}
+
+// Per camera node (wavefront): recompute its observations' node Jacobians and reduce the node
+// diagonal block (full ND x ND) and node rhs. Recomputing is cheaper than storing J.
+template <int FTC, int ND>
+__global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
+                                                       const int32_t* __restrict__ cam_obs_ptr,
+                                                       const int32_t* __restrict__ cam_obs,
+                                                       ObsView ov, const double* __restrict__ values,
+                                                       double* __restrict__ Bdiag,
+                                                       double* __restrict__ gB) {
+  using F = FT<FTC>;
+  constexpr int NH = ND * (ND + 1) / 2;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= n_nodes) return;
+  double h[NH] = {}, g[ND] = {};
+  for (int q0 = cam_obs_ptr[wave] + lane; q0 < cam_obs_ptr[wave + 1]; q0 += 64) {
+    const int o = cam_obs[q0];
+    const double* a[F::NA];
+#pragma unroll
+    for (int k = 0; k < F::NA; ++k) a[k] = values + ov.arg_off[k * ov.n_obs + o];
+    double r[F::D], J[F::D][F::T];
+    F::template eval<double>(a, r, J);
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+#pragma unroll
+      for (int q = 0; q < F::D; ++q) g[i] += J[q][i] * r[q];
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double acc = 0;
+#pragma unroll
+        for (int q = 0; q < F::D; ++q) acc += J[q][i] * J[q][j];
+        h[idx++] += acc;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NH; ++i)
+    for (int off = 32; off > 0; off >>= 1) h[i] += __shfl_down(h[i], off, 64);
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    for (int off = 32; off > 0; off >>= 1) g[i] += __shfl_down(g[i], off, 64);
+  if (lane == 0) {
+    double* Bd = Bdiag + (int64_t)wave * ND * ND;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        Bd[i * ND + j] = h[idx];
+        Bd[j * ND + i] = h[idx];
+        ++idx;
+      }
+#pragma unroll
+    for (int i = 0; i < ND; ++i) gB[(int64_t)wave * ND + i] = g[i];
+  }
+}
+
+// Factors touching only camera nodes (priors, odometry between cameras): thread per factor,
+// H and rhs scattered with atomics into dense per-block storage of the S pattern.
+template <int FTC, int ND>
+__global__ void lin_bfac_kernel(int32_t n, const int32_t* __restrict__ arg_off,
+                                const int32_t* __restrict__ res_off,
+                                const int32_t* __restrict__ col_map,  // [T][n]: node*ND+dim or -1
+                                const int32_t* __restrict__ blk_of,   // [NL*NL][n] block index
+                                const double* __restrict__ values, double* __restrict__ residual,
+                                double* __restrict__ Bfac, double* __restrict__ gB) {
+  using F = FT<FTC>;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const double* a[F::NA];
+#pragma unroll
+  for (int k = 0; k < F::NA; ++k) a[k] = values + arg_off[k * n + f];
+  double r[F::D], J[F::D][F::T];
+  F::template eval<double>(a, r, J);
+  const int ro = res_off[f];
+#pragma unroll
+  for (int i = 0; i < F::D; ++i) residual[ro + i] = r[i];
+  for (int c = 0; c < F::T; ++c) {
+    const int gc = col_map[c * n + f];
+    if (gc < 0) continue;
+    double acc = 0;
+#pragma unroll
+    for (int q = 0; q < F::D; ++q) acc += J[q][c] * r[q];
+    unsafeAtomicAdd(&gB[gc], acc);
+    for (int c2 = 0; c2 < F::T; ++c2) {
+      const int gc2 = col_map[c2 * n + f];
+      if (gc2 < 0) continue;
+      const int na = gc / ND, nb = gc2 / ND;
+      if (na < nb || (na == nb && gc < gc2)) continue;  // lower triangle (node-major)
+      double hh = 0;
+#pragma unroll
+      for (int q = 0; q < F::D; ++q) hh += J[q][c] * J[q][c2];
+      // block (na, nb) entry (row dim gc%ND, col dim gc2%ND)
+      const int blk = blk_of[(c * F::T + c2) * n + f];
+      unsafeAtomicAdd(&Bfac[(int64_t)blk * ND * ND + (gc % ND) * ND + (gc2 % ND)], hh);
+    }
+  }
+}
+
+// Damping of camera and point diagonals (DampHessian semantics, tcc:22-54), writing the damping
+// vector in reference state order.
+template <int ND>
+__global__ void damp_schur_kernel(int32_t n_nodes, int32_t n_points,
+                                  const double* __restrict__ Bdiag, const double* __restrict__ Bfac,
+                                  const int32_t* __restrict__ diag_blk,
+                                  const double* __restrict__ V, const int32_t* __restrict__ node_tan,
+                                  const int32_t* __restrict__ pt_tan,
+                                  const double* __restrict__ lambda_p, int use_diag, int keep_max,
+                                  int have_max, double dmin, int unit, double* __restrict__ max_diag,
+                                  double* __restrict__ damping, double* __restrict__ dnode,
+                                  double* __restrict__ dpoint) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = n_nodes * ND;
+  if (i >= nb + 3 * n_points) return;
+  const double lambda = *lambda_p;
+  double u;
+  int tan;
+  if (i < nb) {
+    const int a = i / ND, k = i % ND;
+    u = Bdiag[(int64_t)a * ND * ND + k * ND + k] + Bfac[(int64_t)diag_blk[a] * ND * ND + k * ND + k];
+    tan = node_tan[i];
+  } else {
+    const int p = (i - nb) / 3, k = (i - nb) % 3;
+    const int di = k == 0 ? 0 : (k == 1 ? 2 : 5);
+    u = V[(int64_t)p * 6 + di];
+    tan = pt_tan[p * 3 + k];
+  }
+  double d = 0;
+  if (use_diag) {
+    if (keep_max) {
+      const double md = have_max ? fmax(max_diag[tan], u) : fmax(u, dmin);
+      max_diag[tan] = md;
+      d = md * lambda;
+    } else {
+      d = fmax(u, dmin) * lambda;
+    }
+  }
+  if (unit) d += lambda;
+  damping[tan] = d;
+  if (i < nb) dnode[i] = d; else dpoint[i - nb] = d;
+}
+
+// Per point: damped V_p -> L_p (Cholesky), h_p = L_p^-1 g_p, Z_o = W_o L_p^-T.
+template <int ND>
+__global__ void __launch_bounds__(128) point_elim_kernel(int32_t n_points,
+                                                         const int32_t* __restrict__ pt_obs_ptr,
+                                                         const double* __restrict__ V,
+                                                         const double* __restrict__ dpoint,
+                                                         const double* __restrict__ gp,
+                                                         const double* __restrict__ W,
+                                                         double* __restrict__ Lp,
+                                                         double* __restrict__ hp,
+                                                         double* __restrict__ Z) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_points) return;
+  const double* v = V + (int64_t)p * 6;
+  const double a00 = v[0] + dpoint[p * 3 + 0], a10 = v[1], a11 = v[2] + dpoint[p * 3 + 1];
+  const double a20 = v[3], a21 = v[4], a22 = v[5] + dpoint[p * 3 + 2];
+  const double l00 = sqrt(a00);
+  const double l10 = a10 / l00, l20 = a20 / l00;
+  const double l11 = sqrt(a11 - l10 * l10);
+  const double l21 = (a21 - l20 * l10) / l11;
+  const double l22 = sqrt(a22 - l20 * l20 - l21 * l21);
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  double* L = Lp + (int64_t)p * 6;
+  L[0] = l00;
+  L[1] = l10;
+  L[2] = l11;
+  L[3] = l20;
+  L[4] = l21;
+  L[5] = l22;
+  const double* g = gp + (int64_t)p * 3;
+  const double h0 = g[0] * i00;
+  const double h1 = (g[1] - l10 * h0) * i11;
+  const double h2 = (g[2] - l20 * h0 - l21 * h1) * i22;
+  hp[p * 3 + 0] = h0;
+  hp[p * 3 + 1] = h1;
+  hp[p * 3 + 2] = h2;
+  // Z = W L^-T: each row w of W solves z L^T = w  (forward substitution with L)
+  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
+    const double* Wo = W + (int64_t)o * ND * 3;
+    double* Zo = Z + (int64_t)o * ND * 3;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      const double z0 = Wo[i * 3 + 0] * i00;
+      const double z1 = (Wo[i * 3 + 1] - l10 * z0) * i11;
+      const double z2 = (Wo[i * 3 + 2] - l20 * z0 - l21 * z1) * i22;
+      Zo[i * 3 + 0] = z0;
+      Zo[i * 3 + 1] = z1;
+      Zo[i * 3 + 2] = z2;
+    }
+  }
+}
+
+// One wavefront per S block (a, b): S = [Bdiag + damping if a == b] + Bfac - sum_pairs Z_a Z_b^T,
+// written into the LDL^T panels at (base + col * ld + row) or transposed.
+template <int ND>
+__global__ void __launch_bounds__(256) rcs_assemble_kernel(
+    int32_t n_blocks, const int32_t* __restrict__ blk_a, const int32_t* __restrict__ blk_b,
+    const int32_t* __restrict__ pair_ptr, const int32_t* __restrict__ pairs,
+    const double* __restrict__ Z, const double* __restrict__ Bdiag,
+    const double* __restrict__ Bfac, const double* __restrict__ dnode,
+    const int64_t* __restrict__ panel_base, const int32_t* __restrict__ panel_ld,
+    const int32_t* __restrict__ panel_trans, double* __restrict__ panel) {
+  constexpr int NE = ND * ND;
+  const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (blk >= n_blocks) return;
+  const int a = blk_a[blk], b = blk_b[blk];
+  // each lane owns entries e = lane and lane + 64 (ND*ND <= 128)
+  double acc[2] = {0, 0};
+  int ei[2], ej[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = lane + 64 * t;
+    ei[t] = e < NE ? e / ND : 0;
+    ej[t] = e < NE ? e % ND : 0;
+  }
+  for (int q = pair_ptr[blk]; q < pair_ptr[blk + 1]; ++q) {
+    const double* Za = Z + (int64_t)pairs[2 * q] * ND * 3;
+    const double* Zb = Z + (int64_t)pairs[2 * q + 1] * ND * 3;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const double* x = Za + ei[t] * 3;
+      const double* y = Zb + ej[t] * 3;
+      acc[t] += x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+    }
+  }
+  const int64_t base = panel_base[blk];
+  const int ld = panel_ld[blk];
+  const bool trans = panel_trans[blk] != 0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = lane + 64 * t;
+    if (e >= NE) continue;
+    const int i = ei[t], j = ej[t];
+    if (a == b && i < j) continue;
+    double s = Bfac[(int64_t)blk * NE + i * ND + j] - acc[t];
+    if (a == b) s += Bdiag[(int64_t)a * NE + i * ND + j] + (i == j ? dnode[a * ND + i] : 0.0);
+    // entry (row a*ND+i, col b*ND+j) of S
+    const int64_t pos = trans ? base + (int64_t)i * ld + j : base + (int64_t)j * ld + i;
+    panel[pos] = s;
+  }
+}
+
+// Per camera node (wavefront): v_a = g_a - sum_{o in a} Z_o h_p(o).
+template <int ND>
+__global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
+                                                          const int32_t* __restrict__ cam_obs_ptr,
+                                                          const int32_t* __restrict__ cam_obs,
+                                                          const int32_t* __restrict__ obs_point,
+                                                          const double* __restrict__ Z,
+                                                          const double* __restrict__ hp,
+                                                          const double* __restrict__ gB,
+                                                          double* __restrict__ v) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= n_nodes) return;
+  double acc[ND] = {};
+  for (int q = cam_obs_ptr[wave] + lane; q < cam_obs_ptr[wave + 1]; q += 64) {
+    const int o = cam_obs[q];
+    const int p = obs_point[o];
+    const double h0 = hp[p * 3], h1 = hp[p * 3 + 1], h2 = hp[p * 3 + 2];
+    const double* Zo = Z + (int64_t)o * ND * 3;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) acc[i] += Zo[i * 3] * h0 + Zo[i * 3 + 1] * h1 + Zo[i * 3 + 2] * h2;
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_down(acc[i], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < ND; ++i) v[wave * ND + i] = gB[wave * ND + i] - acc[i];
+}
+
+// Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
+// update vector in reference state order.
+template <int ND>
+__global__ void __launch_bounds__(128) back_substitute_kernel(
+    int32_t n_points, const int32_t* __restrict__ pt_obs_ptr, const int32_t* __restrict__ obs_node,
+    const double* __restrict__ Z, const double* __restrict__ Lp, const double* __restrict__ hp,
+    const double* __restrict__ y, const int32_t* __restrict__ pt_tan, double* __restrict__ update) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_points) return;
+  double t0 = hp[p * 3], t1 = hp[p * 3 + 1], t2 = hp[p * 3 + 2];
+  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
+    const double* Zo = Z + (int64_t)o * ND * 3;
+    const double* yn = y + obs_node[o] * ND;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      t0 -= Zo[i * 3] * yn[i];
+      t1 -= Zo[i * 3 + 1] * yn[i];
+      t2 -= Zo[i * 3 + 2] * yn[i];
+    }
+  }
+  const double* L = Lp + (int64_t)p * 6;
+  const double z2 = t2 / L[5];
+  const double z1 = (t1 - L[4] * z2) / L[2];
+  const double z0 = (t0 - L[1] * z1 - L[3] * z2) / L[0];
+  update[pt_tan[p * 3 + 0]] = -z0;
+  update[pt_tan[p * 3 + 1]] = -z1;
+  update[pt_tan[p * 3 + 2]] = -z2;
+}
+
+__global__ void scatter_neg_kernel(int32_t n, const int32_t* __restrict__ idx,
+                                   const double* __restrict__ x, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = -x[i];
+}
+__global__ void scatter_kernel(int32_t n, const int32_t* __restrict__ idx,
+                               const double* __restrict__ x, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[idx[i]] = x[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+class SchurBackend final : public SystemBackend {
+ public:
+  SchurBackend(const HostProblem& p, const SolverOptions& o, hipStream_t s);
+  void relinearize(int b, const double* values, double* err_out, hipStream_t s) override;
+  void damp(int b, const double* lambda_dev, const symhip_optimizer_params& p,
+            bool have_max_diagonal, hipStream_t s) override;
+  void restore(int, hipStream_t) override {}  // damping is applied out of place
+  void solve(int b, double* update, hipStream_t s) override;
+  const double* rhs(int b) const override { return rhs_[b].get(); }
+  const double* residual(int b) const override { return residual_[b].get(); }
+  const double* damping() const override { return damping_.get(); }
+  void hessian_csc(int b, double* out, hipStream_t s) override;
+  int32_t N() const override { return N_; }
+  int32_t M() const override { return M_; }
+  int64_t nnz() const override { return host_->nnz(); }
+  void info(int64_t* out) const override {
+    const CholSymbolic& y = chol_.symbolic();
+    out[0] = y.n;
+    out[1] = y.n_snodes;
+    out[2] = y.panel_size();
+    out[3] = y.n_levels;
+    out[4] = (int64_t)y.factor_flops;
+    out[5] = n_blocks_;
+    out[6] = n_pairs_;
+    out[7] = n_obs_;
+  }
+
+ private:
+  template <int ND>
+  void solve_nd(int b, double* update, hipStream_t s);
+  template <int ND>
+  void relin_nd(int b, const double* values, hipStream_t s);
+
+  const HostProblem* host_;
+  int nd_ = 0, ftype_ = 0;
+  int32_t n_nodes_ = 0, n_points_ = 0, n_obs_ = 0, n_blocks_ = 0, N_ = 0, M_ = 0;
+  int64_t n_pairs_ = 0;
+  // observation tables
+  DBuf<int32_t> obs_arg_off_, obs_res_off_, obs_node_, obs_point_, pt_obs_ptr_, cam_obs_ptr_,
+      cam_obs_;
+  DBuf<int32_t> node_tan_, pt_tan_, blk_a_, blk_b_, pair_ptr_, pairs_, diag_blk_;
+  DBuf<int64_t> panel_base_;
+  DBuf<int32_t> panel_ld_, panel_trans_;
+  // camera-only factors
+  struct BGroup {
+    int type;
+    int32_t n;
+    DBuf<int32_t> arg_off, res_off, col_map, blk_of;
+  };
+  std::vector<BGroup> bgroups_;
+  // per state block storage
+  DBuf<double> residual_[3], W_[3], V_[3], gp_[3], Bdiag_[3], Bfac_[3], gB_[3], rhs_[3];
+  DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, v_, y_, partials_;
+  SupernodalLDLT chol_;
+  std::unique_ptr<CscLinearizer> csc_;  // reference-layout H for best-linearization output
+};
+
+namespace {
+struct UnionFind {
+  std::vector<int32_t> p;
+  explicit UnionFind(int n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+  int find(int x) {
+    while (p[x] != x) x = p[x] = p[p[x]];
+    return x;
+  }
+  void unite(int a, int b) { p[find(a)] = find(b); }
+};
+}  // namespace
+
+SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStream_t s)
+    : host_(&P) {
+  const int32_t S = (int32_t)P.state_keys.size();
+  N_ = P.N;
+  M_ = P.M;
+  std::vector<int32_t> pos_of(P.n_keys, -1);
+  for (int32_t i = 0; i < S; ++i) pos_of[P.state_keys[i]] = i;
+  // Landmarks: trailing VECTOR3 state keys that no factor touches twice.
+  std::vector<int32_t> touch_count(S, 0);
+  std::vector<std::vector<int32_t>> fpos(P.n_factors);  // state positions of linearized args
+  for (int32_t f = 0; f < P.n_factors; ++f) {
+    const FactorSpec sp = factor_spec(P.ftype[f]);
+    for (int a = 0; a < sp.nargs; ++a)
+      if ((sp.lin_mask >> a) & 1) {
+        const int32_t k = pos_of[P.fargs[P.fargs_ptr[f] + a]];
+        SYMHIP_CHECK(k >= 0, "Schur solver requires every linearized argument to be optimized");
+        fpos[f].push_back(k);
+      }
+  }
+  int32_t first_c = S;
+  {
+    std::vector<char> is_v3(S);
+    for (int32_t i = 0; i < S; ++i) is_v3[i] = P.types[P.state_keys[i]] == SYMHIP_TYPE_VECTOR3;
+    while (first_c > 0 && is_v3[first_c - 1]) --first_c;
+    // shrink until no factor touches two landmark keys
+    bool ok = false;
+    while (!ok) {
+      ok = true;
+      for (int32_t f = 0; f < P.n_factors && ok; ++f) {
+        int cnt = 0, minc = S;
+        for (int32_t k : fpos[f])
+          if (k >= first_c) {
+            ++cnt;
+            minc = std::min(minc, k);
+          }
+        if (cnt > 1) {
+          first_c = minc + 1;
+          ok = false;
+        }
+      }
+    }
+  }
+  n_points_ = S - first_c;
+  SYMHIP_CHECK(n_points_ > 0, "Schur solver: no landmark block found at the end of the state");
+  // Camera nodes: union B keys co-occurring in landmark factors.
+  UnionFind uf(first_c);
+  std::vector<int32_t> lm_factors, b_factors;
+  for (int32_t f = 0; f < P.n_factors; ++f) {
+    bool lm = false;
+    for (int32_t k : fpos[f]) lm |= k >= first_c;
+    if (lm) {
+      lm_factors.push_back(f);
+      int32_t first_b = -1;
+      for (int32_t k : fpos[f])
+        if (k < first_c) {
+          if (first_b < 0) first_b = k; else uf.unite(k, first_b);
+        }
+    } else {
+      b_factors.push_back(f);
+    }
+  }
+  SYMHIP_CHECK(!lm_factors.empty(), "Schur solver: no landmark factors");
+  ftype_ = P.ftype[lm_factors[0]];
+  for (int32_t f : lm_factors)
+    SYMHIP_CHECK(P.ftype[f] == ftype_, "Schur solver: one landmark factor type per problem");
+  SYMHIP_CHECK(ftype_ == SYMHIP_FACTOR_SNAVELY || ftype_ == SYMHIP_FACTOR_SNAVELY_HUBER ||
+                   ftype_ == SYMHIP_FACTOR_REPROJ_CAUCHY,
+               "Schur solver: unsupported landmark factor type");
+  // node ids in order of their first state key; node layout = member keys in state order
+  std::vector<int32_t> node_of_pos(first_c, -1), root_node;
+  std::vector<int32_t> root_to_node(first_c, -1);
+  std::vector<std::vector<int32_t>> node_members;
+  for (int32_t k = 0; k < first_c; ++k) {
+    const int r = uf.find(k);
+    if (root_to_node[r] < 0) {
+      root_to_node[r] = (int32_t)node_members.size();
+      node_members.push_back({});
+    }
+    node_of_pos[k] = root_to_node[r];
+    node_members[root_to_node[r]].push_back(k);
+  }
+  n_nodes_ = (int32_t)node_members.size();
+  nd_ = 0;
+  std::vector<int32_t> pos_off_in_node(first_c, 0);
+  for (auto& mem : node_members) {
+    int d = 0;
+    for (int32_t k : mem) {
+      pos_off_in_node[k] = d;
+      d += tangent_dim(P.types[P.state_keys[k]]);
+    }
+    if (nd_ == 0) nd_ = d;
+    SYMHIP_CHECK(d == nd_, "Schur solver: camera nodes must all have the same dimension");
+  }
+  const FactorSpec lsp = factor_spec(ftype_);
+  SYMHIP_CHECK(nd_ == lsp.tdim - 3, "Schur solver: camera node does not match the factor layout");
+  // each landmark factor: B args in node layout order, landmark last
+  for (int32_t f : lm_factors) {
+    const auto& ks = fpos[f];
+    SYMHIP_CHECK(ks.size() >= 2 && ks[0] < first_c,
+                 "Schur solver: landmark factors must also touch a camera node");
+    SYMHIP_CHECK(ks.back() >= first_c, "Schur solver: landmark must be the last linearized arg");
+    int off = 0;
+    for (size_t i = 0; i + 1 < ks.size(); ++i) {
+      SYMHIP_CHECK(node_of_pos[ks[i]] == node_of_pos[ks[0]] && pos_off_in_node[ks[i]] == off,
+                   "Schur solver: factor arguments do not follow the camera node layout");
+      off += tangent_dim(P.types[P.state_keys[ks[i]]]);
+    }
+  }
+  // node -> reference tangent offsets
+  std::vector<int32_t> node_tan((size_t)n_nodes_ * nd_);
+  for (int32_t a = 0; a < n_nodes_; ++a)
+    for (int32_t k : node_members[a]) {
+      const int td = tangent_dim(P.types[P.state_keys[k]]);
+      for (int d = 0; d < td; ++d)
+        node_tan[(size_t)a * nd_ + pos_off_in_node[k] + d] = P.tan_off[P.state_keys[k]] + d;
+    }
+  // observations grouped by point; points sorted by their first camera node
+  std::vector<std::vector<int32_t>> obs_of_pt(n_points_);
+  std::vector<int32_t> obs_node_raw(P.n_factors, -1);
+  for (int32_t f : lm_factors) {
+    obs_of_pt[fpos[f].back() - first_c].push_back(f);
+    obs_node_raw[f] = node_of_pos[fpos[f][0]];
+  }
+  std::vector<int32_t> pt_order(n_points_);
+  std::iota(pt_order.begin(), pt_order.end(), 0);
+  std::vector<int32_t> first_node(n_points_, INT32_MAX);
+  for (int32_t p = 0; p < n_points_; ++p) {
+    SYMHIP_CHECK(!obs_of_pt[p].empty(), "Schur solver: landmark without observations");
+    std::sort(obs_of_pt[p].begin(), obs_of_pt[p].end(),
+              [&](int32_t x, int32_t y) { return obs_node_raw[x] < obs_node_raw[y]; });
+    first_node[p] = obs_node_raw[obs_of_pt[p][0]];
+  }
+  std::stable_sort(pt_order.begin(), pt_order.end(),
+                   [&](int32_t x, int32_t y) { return first_node[x] < first_node[y]; });
+  std::vector<int32_t> obs_f, pt_ptr{0}, obs_node, obs_point, pt_tan((size_t)n_points_ * 3);
+  for (int32_t i = 0; i < n_points_; ++i) {
+    const int32_t p = pt_order[i];
+    for (int32_t f : obs_of_pt[p]) {
+      obs_f.push_back(f);
+      obs_node.push_back(obs_node_raw[f]);
+      obs_point.push_back(i);
+    }
+    pt_ptr.push_back((int32_t)obs_f.size());
+    for (int d = 0; d < 3; ++d) pt_tan[(size_t)i * 3 + d] = P.tan_off[P.state_keys[first_c + p]] + d;
+  }
+  n_obs_ = (int32_t)obs_f.size();
+  std::vector<int32_t> arg_off((size_t)lsp.nargs * n_obs_), res_off(n_obs_);
+  for (int32_t o = 0; o < n_obs_; ++o) {
+    const int32_t f = obs_f[o];
+    for (int a = 0; a < lsp.nargs; ++a)
+      arg_off[(size_t)a * n_obs_ + o] = P.storage_offset[P.fargs[P.fargs_ptr[f] + a]];
+    res_off[o] = P.res_off[f];
+  }
+  std::vector<int32_t> cam_ptr(n_nodes_ + 1, 0), cam_obs(n_obs_);
+  for (int32_t o = 0; o < n_obs_; ++o) cam_ptr[obs_node[o] + 1]++;
+  for (int32_t a = 0; a < n_nodes_; ++a) cam_ptr[a + 1] += cam_ptr[a];
+  {
+    std::vector<int32_t> fill(cam_ptr.begin(), cam_ptr.end() - 1);
+    for (int32_t o = 0; o < n_obs_; ++o) cam_obs[fill[obs_node[o]]++] = o;
+  }
+  // S block pattern: diagonal, covisible pairs, camera-factor couplings. Lower: a >= b.
+  std::vector<std::vector<int32_t>> row_cols(n_nodes_);
+  for (int32_t a = 0; a < n_nodes_; ++a) row_cols[a].push_back(a);
+  for (int32_t p = 0; p < n_points_; ++p)
+    for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
+      for (int32_t y = pt_ptr[p]; y < x; ++y) {
+        const int32_t a = std::max(obs_node[x], obs_node[y]), b = std::min(obs_node[x], obs_node[y]);
+        row_cols[a].push_back(b);
+      }
+  for (int32_t f : b_factors)
+    for (int32_t k1 : fpos[f])
+      for (int32_t k2 : fpos[f]) {
+        const int32_t a = node_of_pos[k1], b = node_of_pos[k2];
+        if (a >= b) row_cols[a].push_back(b);
+      }
+  std::vector<int32_t> row_ptr(n_nodes_ + 1, 0), blk_a, blk_b;
+  for (int32_t a = 0; a < n_nodes_; ++a) {
+    auto& rc = row_cols[a];
+    std::sort(rc.begin(), rc.end());
+    rc.erase(std::unique(rc.begin(), rc.end()), rc.end());
+    row_ptr[a + 1] = row_ptr[a] + (int32_t)rc.size();
+    for (int32_t b : rc) {
+      blk_a.push_back(a);
+      blk_b.push_back(b);
+    }
+  }
+  n_blocks_ = (int32_t)blk_a.size();
+  auto block_index = [&](int32_t a, int32_t b) -> int32_t {
+    const auto& rc = row_cols[a];
+    const auto it = std::lower_bound(rc.begin(), rc.end(), b);
+    SYMHIP_CHECK(it != rc.end() && *it == b, "internal: S block missing");
+    return row_ptr[a] + (int32_t)(it - rc.begin());
+  };
+  std::vector<int32_t> diag_blk(n_nodes_);
+  for (int32_t a = 0; a < n_nodes_; ++a) diag_blk[a] = block_index(a, a);
+  // Observation pairs per block
+  std::vector<int32_t> pcount(n_blocks_ + 1, 0);
+  for (int32_t p = 0; p < n_points_; ++p)
+    for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
+      for (int32_t y = pt_ptr[p]; y < pt_ptr[p + 1]; ++y) {
+        const int32_t a = obs_node[x], b = obs_node[y];
+        if (a >= b) pcount[block_index(a, b) + 1] += 1;
+      }
+  for (int32_t k = 0; k < n_blocks_; ++k) pcount[k + 1] += pcount[k];
+  n_pairs_ = pcount[n_blocks_];
+  std::vector<int32_t> pairs(2 * (size_t)pcount[n_blocks_]);
+  {
+    std::vector<int32_t> fill(pcount.begin(), pcount.end() - 1);
+    for (int32_t p = 0; p < n_points_; ++p)
+      for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
+        for (int32_t y = pt_ptr[p]; y < pt_ptr[p + 1]; ++y) {
+          const int32_t a = obs_node[x], b = obs_node[y];
+          if (a < b) continue;
+          const int32_t q = fill[block_index(a, b)]++;
+          pairs[2 * (size_t)q] = x;
+          pairs[2 * (size_t)q + 1] = y;
+        }
+  }
+  // RCS symbolic: nodes = cameras (uniform nd), lower CSC pattern of the dense blocks.
+  const int32_t nB = n_nodes_ * nd_;
+  std::vector<int32_t> scp(nB + 1, 0), sri;
+  {
+    // column (b, j): rows (b, i>=j) then all rows of a > b in block column b
+    std::vector<std::vector<int32_t>> col_rows(n_nodes_);
+    for (int32_t k = 0; k < n_blocks_; ++k)
+      if (blk_a[k] != blk_b[k]) col_rows[blk_b[k]].push_back(blk_a[k]);
+    for (auto& v : col_rows) std::sort(v.begin(), v.end());
+    for (int32_t b = 0; b < n_nodes_; ++b)
+      for (int j = 0; j < nd_; ++j) {
+        for (int i = j; i < nd_; ++i) sri.push_back(b * nd_ + i);
+        for (int32_t a : col_rows[b])
+          for (int i = 0; i < nd_; ++i) sri.push_back(a * nd_ + i);
+        scp[b * nd_ + j + 1] = (int32_t)sri.size();
+      }
+  }
+  std::vector<int32_t> node_ptr(n_nodes_ + 1);
+  for (int32_t a = 0; a <= n_nodes_; ++a) node_ptr[a] = a * nd_;
+  CholSymbolic sym;
+  chol_symbolic(nB, scp.data(), sri.data(), node_ptr,
+                o.ordering == 0 ? Ordering::kNatural : Ordering::kMinDegree, nullptr, 64, sym);
+  // panel position of each S block's (0,0) entry
+  std::vector<int32_t> sn_of_col(nB);
+  for (int32_t k = 0; k < sym.n_snodes; ++k)
+    for (int32_t c = sym.sn_col_begin[k]; c < sym.sn_col_begin[k + 1]; ++c) sn_of_col[c] = k;
+  std::vector<int64_t> pbase(n_blocks_);
+  std::vector<int32_t> pld(n_blocks_), ptrans(n_blocks_);
+  for (int32_t k = 0; k < n_blocks_; ++k) {
+    int32_t ra = sym.perm[blk_a[k] * nd_], cb = sym.perm[blk_b[k] * nd_];
+    const bool trans = ra < cb;
+    if (trans) std::swap(ra, cb);
+    const int32_t t = sn_of_col[cb];
+    const int32_t c0 = sym.sn_col_begin[t], c1 = sym.sn_col_begin[t + 1];
+    int32_t rp;
+    if (ra < c1) {
+      rp = ra - c0;
+    } else {
+      const int32_t* b0 = sym.sn_rows.data() + sym.sn_rows_ptr[t] + (c1 - c0);
+      const int32_t* e0 = sym.sn_rows.data() + sym.sn_rows_ptr[t + 1];
+      const int32_t* it = std::lower_bound(b0, e0, ra);
+      SYMHIP_CHECK(it != e0 && *it == ra, "internal: S block outside the factor pattern");
+      rp = (int32_t)(it - (sym.sn_rows.data() + sym.sn_rows_ptr[t]));
+    }
+    const int32_t ld = sym.sn_rows_ptr[t + 1] - sym.sn_rows_ptr[t];
+    pbase[k] = sym.sn_panel_off[t] + (int64_t)(cb - c0) * ld + rp;
+    pld[k] = ld;
+    ptrans[k] = trans ? 1 : 0;
+  }
+  chol_.setup(sym, scp.data(), sri.data(), s);
+  // Camera-only factors
+  std::vector<std::vector<int32_t>> bf_by_type(kNumFactorTypes);
+  for (int32_t f : b_factors) bf_by_type[P.ftype[f]].push_back(f);
+  for (int t = 1; t < kNumFactorTypes; ++t) {
+    if (bf_by_type[t].empty()) continue;
+    const FactorSpec sp = factor_spec(t);
+    const int32_t n = (int32_t)bf_by_type[t].size();
+    std::vector<int32_t> ao((size_t)sp.nargs * n), ro(n), cm((size_t)sp.tdim * n),
+        bo((size_t)sp.tdim * sp.tdim * n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t f = bf_by_type[t][i];
+      for (int a = 0; a < sp.nargs; ++a)
+        ao[(size_t)a * n + i] = P.storage_offset[P.fargs[P.fargs_ptr[f] + a]];
+      ro[i] = P.res_off[f];
+      int c = 0;
+      for (int32_t k : fpos[f]) {
+        const int td = tangent_dim(P.types[P.state_keys[k]]);
+        for (int d = 0; d < td; ++d, ++c)
+          cm[(size_t)c * n + i] = node_of_pos[k] * nd_ + pos_off_in_node[k] + d;
+      }
+      for (int c1 = 0; c1 < sp.tdim; ++c1)
+        for (int c2 = 0; c2 < sp.tdim; ++c2) {
+          const int32_t na = cm[(size_t)c1 * n + i] / nd_, nb = cm[(size_t)c2 * n + i] / nd_;
+          if (na >= nb) bo[((size_t)c1 * sp.tdim + c2) * n + i] = block_index(na, nb);
+        }
+    }
+    BGroup g;
+    g.type = t;
+    g.n = n;
+    g.arg_off.upload(ao, s);
+    g.res_off.upload(ro, s);
+    g.col_map.upload(cm, s);
+    g.blk_of.upload(bo, s);
+    bgroups_.push_back(std::move(g));
+  }
+  obs_arg_off_.upload(arg_off, s);
+  obs_res_off_.upload(res_off, s);
+  obs_node_.upload(obs_node, s);
+  obs_point_.upload(obs_point, s);
+  pt_obs_ptr_.upload(pt_ptr, s);
+  cam_obs_ptr_.upload(cam_ptr, s);
+  cam_obs_.upload(cam_obs, s);
+  node_tan_.upload(node_tan, s);
+  pt_tan_.upload(pt_tan, s);
+  blk_a_.upload(blk_a, s);
+  blk_b_.upload(blk_b, s);
+  pair_ptr_.upload(pcount, s);
+  pairs_.upload(pairs, s);
+  diag_blk_.upload(diag_blk, s);
+  panel_base_.upload(pbase, s);
+  panel_ld_.upload(pld, s);
+  panel_trans_.upload(ptrans, s);
+  const int64_t nde = (int64_t)nd_ * nd_;
+  for (int b = 0; b < 3; ++b) {
+    residual_[b].resize(std::max(1, M_));
+    W_[b].resize((size_t)n_obs_ * nd_ * 3);
+    V_[b].resize((size_t)n_points_ * 6);
+    gp_[b].resize((size_t)n_points_ * 3);
+    Bdiag_[b].resize((size_t)n_nodes_ * nde);
+    Bfac_[b].resize((size_t)n_blocks_ * nde);
+    gB_[b].resize((size_t)n_nodes_ * nd_);
+    rhs_[b].resize(std::max(1, N_));
+  }
+  damping_.resize(std::max(1, N_));
+  max_diag_.resize(std::max(1, N_));
+  dnode_.resize((size_t)n_nodes_ * nd_);
+  dpoint_.resize((size_t)n_points_ * 3);
+  Lp_.resize((size_t)n_points_ * 6);
+  hp_.resize((size_t)n_points_ * 3);
+  Z_.resize((size_t)n_obs_ * nd_ * 3);
+  v_.resize((size_t)n_nodes_ * nd_);
+  y_.resize((size_t)n_nodes_ * nd_);
+  partials_.resize(kReduceBlocks);
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+template <int ND>
+void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
+  const ObsView ov{n_obs_, obs_arg_off_.get(), obs_res_off_.get(), obs_node_.get()};
+  dispatch_factor_type(ftype_, [&]<int C>() {
+    if constexpr (FT<C>::T == ND + 3 && FT<C>::NL >= 2) {
+      {
+        PhaseScope ps(timer_, SYMHIP_PHASE_LIN_POINTS, s);
+        hipLaunchKernelGGL((lin_points_kernel<C, ND>), dim3(grid_for(n_points_, 128)), dim3(128),
+                           0, s, n_points_, pt_obs_ptr_.get(), ov, values, residual_[b].get(),
+                           W_[b].get(), V_[b].get(), gp_[b].get());
+      }
+      PhaseScope ps(timer_, SYMHIP_PHASE_LIN_CAMS, s);
+      hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
+                         dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(), ov, values,
+                         Bdiag_[b].get(), gB_[b].get());
+    }
+  });
+  fill_zero(Bfac_[b].get(), (int64_t)n_blocks_ * ND * ND, s);
+  for (const BGroup& g : bgroups_) {
+    dispatch_factor_type(g.type, [&]<int C>() {
+      hipLaunchKernelGGL((lin_bfac_kernel<C, ND>), dim3(grid_for(g.n, 64)), dim3(64), 0, s, g.n,
+                         g.arg_off.get(), g.res_off.get(), g.col_map.get(), g.blk_of.get(), values,
+                         residual_[b].get(), Bfac_[b].get(), gB_[b].get());
+    });
+  }
+  // rhs in reference order
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(n_nodes_ * ND, 256)), dim3(256), 0, s,
+                     n_nodes_ * ND, node_tan_.get(), gB_[b].get(), rhs_[b].get());
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(n_points_ * 3, 256)), dim3(256), 0, s,
+                     n_points_ * 3, pt_tan_.get(), gp_[b].get(), rhs_[b].get());
+  HIP_OK(hipGetLastError());
+}
+
+void SchurBackend::relinearize(int b, const double* values, double* err_out, hipStream_t s) {
+  if (nd_ == 9) relin_nd<9>(b, values, s); else relin_nd<6>(b, values, s);
+  half_squared_norm(residual_[b].get(), M_, partials_.get(), err_out, s);
+}
+
+void SchurBackend::damp(int b, const double* lambda_dev, const symhip_optimizer_params& p,
+                        bool have_max, hipStream_t s) {
+  const int32_t n = n_nodes_ * nd_ + 3 * n_points_;
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n_nodes_, n_points_,
+                       Bdiag_[b].get(), Bfac_[b].get(), diag_blk_.get(), V_[b].get(),
+                       node_tan_.get(), pt_tan_.get(), lambda_dev, p.use_diagonal_damping,
+                       p.keep_max_diagonal_damping, have_max ? 1 : 0, p.diagonal_damping_min,
+                       p.use_unit_damping, max_diag_.get(), damping_.get(), dnode_.get(),
+                       dpoint_.get());
+  };
+  if (nd_ == 9) launch(damp_schur_kernel<9>); else launch(damp_schur_kernel<6>);
+}
+
+template <int ND>
+void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
+  {
+    PhaseScope ps(timer_, SYMHIP_PHASE_POINT_ELIM, s);
+    hipLaunchKernelGGL((point_elim_kernel<ND>), dim3(grid_for(n_points_, 128)), dim3(128), 0, s,
+                       n_points_, pt_obs_ptr_.get(), V_[b].get(), dpoint_.get(), gp_[b].get(),
+                       W_[b].get(), Lp_.get(), hp_.get(), Z_.get());
+  }
+  fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
+  {
+    PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);
+    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(grid_for((int64_t)n_blocks_ * 64, 256)),
+                       dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
+                       pairs_.get(), Z_.get(), Bdiag_[b].get(), Bfac_[b].get(), dnode_.get(),
+                       panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels());
+  }
+  {
+    PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
+    hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
+                       dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
+                       obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get());
+  }
+  {
+    PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
+    chol_.factorize_assembled(s);
+  }
+  {
+    PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
+    chol_.solve(v_.get(), y_.get(), s);
+  }
+  PhaseScope ps(timer_, SYMHIP_PHASE_BACK_SUB, s);
+  hipLaunchKernelGGL(scatter_neg_kernel, dim3(grid_for(n_nodes_ * ND, 256)), dim3(256), 0, s,
+                     n_nodes_ * ND, node_tan_.get(), y_.get(), update);
+  hipLaunchKernelGGL((back_substitute_kernel<ND>), dim3(grid_for(n_points_, 128)), dim3(128), 0, s,
+                     n_points_, pt_obs_ptr_.get(), obs_node_.get(), Z_.get(), Lp_.get(), hp_.get(),
+                     y_.get(), pt_tan_.get(), update);
+  HIP_OK(hipGetLastError());
+}
+
+void SchurBackend::solve(int b, double* update, hipStream_t s) {
+  if (nd_ == 9) solve_nd<9>(b, update, s); else solve_nd<6>(b, update, s);
+}
+
+void SchurBackend::hessian_csc(int b, double* out, hipStream_t s) {
+  (void)b;
+  (void)out;
+  (void)s;
+  throw Error("symhip: reference-layout Hessian output is not available on the Schur path");
+}
+
+std::unique_ptr<SystemBackend> make_schur_backend(const HostProblem& p, const SolverOptions& o,
+                                                  hipStream_t s) {
+  return std::make_unique<SchurBackend>(p, o, s);
 }
 
 }  // namespace symhip

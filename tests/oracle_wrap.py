@@ -54,6 +54,10 @@ def lib():
                                       C.c_int32, C.c_double, C.c_int32, capi.F64P,
                                       C.POINTER(capi.IterationStats), C.c_int32,
                                       C.POINTER(capi.OptimizationStats), capi.F64P]
+        L.oracle_optimize_timed.argtypes = [vp, C.POINTER(capi.OptimizerParams), C.c_int,
+                                            capi.I32P, C.c_int32, C.c_double, C.c_int32,
+                                            capi.F64P, C.POINTER(capi.IterationStats), C.c_int32,
+                                            C.POINTER(capi.OptimizationStats), capi.F64P]
         L.oracle_linearize_factor.argtypes = [C.c_int32, C.POINTER(capi.F64P), capi.F64P,
                                               capi.F64P, capi.F64P, capi.F64P]
         _lib = L
@@ -139,6 +143,21 @@ class OracleProblem:
         if record_updates:
             out["updates"] = upd[: (st.num_iterations - 1) * self.N].reshape(-1, self.N)
         return out
+
+
+def time_iterations(op: "OracleProblem", params, num_iterations, solver=CHOLESKY, c_dim=0):
+    """Runs num_iterations LM iterations and returns the seconds spent in each Iterate()."""
+    vals = op.prob.values.copy()
+    iters = (capi.IterationStats * (num_iterations + 2))()
+    st = capi.OptimizationStats()
+    secs = np.zeros(num_iterations)
+    eps = op.prob.info.get("epsilon", 2.220446049250313e-15)
+    rc = lib().oracle_optimize_timed(op.h, C.byref(params), solver, None, c_dim, eps,
+                                     num_iterations, _p(vals, C.c_double), iters,
+                                     num_iterations + 2, C.byref(st), _p(secs, C.c_double))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return secs, capi.iteration_records(iters, st.num_iterations)
 
 
 class OracleCholesky:
