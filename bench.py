@@ -158,7 +158,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the optimizer path runs on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md but not built). Use --gpus 1.",
+              "(landmark sharding over several GPUs is designed but not built). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     if capi.hip_lib().symhip_device_count() < 1:

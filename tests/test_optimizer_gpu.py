@@ -25,7 +25,9 @@ def assert_same_trajectory(hip, ora, rtol=RTOL):
     for a, b in zip(hip["iterations"], ora["iterations"]):
         assert a["iteration"] == b["iteration"]
         assert a["update_accepted"] == b["update_accepted"]
-        assert a["current_lambda"] == pytest.approx(b["current_lambda"], rel=1e-12)
+        # DYNAMIC lambda multiplies by a function of the gain ratio, a quotient of two small
+        # differences, so ~1e-15 cost differences move lambda at ~1e-11 relative
+        assert a["current_lambda"] == pytest.approx(b["current_lambda"], rel=1e-8)
         for k in ("new_error", "new_error_linear"):
             assert a[k] == pytest.approx(b[k], rel=rtol, abs=1e-12), (k, a, b)
 
@@ -139,7 +141,7 @@ def test_status_initial_error_not_finite():
 
 @pytest.mark.parametrize("name,gen,params", [
     ("ba_circle", lambda: capi.gen_ba_circle(8, 150, 3), dict()),
-    ("pose_graph", lambda: capi.gen_pose_graph(400, 80, 7), dict()),
+    ("pose_graph", lambda: capi.gen_pose_graph(2000, 150, 7), dict()),
     ("localization", lambda: capi.gen_localization(30, 25, 0.3, 5), dict()),
     ("ba_snavely_cholesky", lambda: capi.gen_ba_snavely(10, 200, 5),
      dict(lambda_update_type=capi.LAMBDA_DYNAMIC)),

@@ -14,7 +14,7 @@ CASES = {
     "robot3d": lambda: pb.robot3d_problem()[0],
     "ba_circle": lambda: capi.gen_ba_circle(6, 60, 3),
     "ba_snavely": lambda: capi.gen_ba_snavely(12, 80, 5),
-    "pose_graph": lambda: capi.gen_pose_graph(300, 60, 7),
+    "pose_graph": lambda: capi.gen_pose_graph(2000, 150, 7),
     "localization": lambda: capi.gen_localization(12, 15, 0.3, 9),
 }
 
