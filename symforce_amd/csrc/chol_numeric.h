@@ -25,16 +25,16 @@ class SupernodalLDLT {
 
   double* panels() const { return panel_.get(); }
   const CholSymbolic& symbolic() const { return sym_; }
-  // Downloads L (dense panels) and D for checking.
+  // Downloads the panels (L21 below the diagonal blocks; the diagonal blocks keep A11) and D.
   void download_factor(std::vector<double>& panels, std::vector<double>& d, hipStream_t s) const;
   int64_t assembly_nnz() const { return (int64_t)asm_map_.size(); }
 
  private:
   CholSymbolic sym_;
-  DBuf<double> panel_, D_, y_;
-  DBuf<int64_t> asm_map_, panel_off_;
-  DBuf<int32_t> col_begin_, rows_ptr_, rows_, perm_;
-  DBuf<int32_t> upd_ptr_, upd_target_, upd_seg_begin_, upd_rel_ptr_, upd_rel_, upd_col_off_;
+  DBuf<double> panel_, l11_, D_, y_;
+  DBuf<int64_t> asm_map_, scat_col_;
+  DBuf<int32_t> rows_, perm_, upd_rel_, scat_rel_;
+  DBuf<uint8_t> desc_;  // per-supernode descriptors (chol_numeric.hip)
   // work lists (flattened over levels)
   std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_;
   DBuf<int32_t> lvl_snodes_, trsm_items_, upd_items_;

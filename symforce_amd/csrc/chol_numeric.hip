@@ -1,6 +1,12 @@
-// Device numeric supernodal LDL^T: assembly, per-level diagonal-block factorization, panel
+// Device numeric supernodal LDL^T: assembly, per-level fused diagonal-block factorization + panel
 // triangular solve, right-looking update (SYRK-like) scattered into ancestor panels, and the
 // triangular solves. All FP64.
+//
+// The work per supernode is small (panels are at most 64 columns wide), so every kernel is
+// latency-bound: the kernels are written to avoid data-dependent branches (the 64 elimination
+// steps are fully unrolled over a panel padded to 64 columns, the padding being an identity
+// block) and chains of dependent global loads (each supernode's metadata is one descriptor,
+// scatter targets are precomputed per below-row).
 #include <algorithm>
 
 #include "./chol_numeric.h"
@@ -10,16 +16,15 @@ namespace symhip {
 
 namespace {
 
-constexpr int kMaxCols = 64;  // supernode width cap: diag block fits in LDS (32 KB)
-constexpr int kTile = 64;     // update tile
+constexpr int kMaxCols = 64;  // supernode width cap (panels padded to this width)
+constexpr int kTile = 64;     // update tile (rows)
 
-struct SnView {
-  const int64_t* panel_off;
-  const int32_t* col_begin;
-  const int32_t* rows_ptr;
-  const int32_t* rows;
-  __device__ int ncols(int s) const { return col_begin[s + 1] - col_begin[s]; }
-  __device__ int ld(int s) const { return rows_ptr[s + 1] - rows_ptr[s]; }
+// l11_off: the factored diagonal block L11 (n x n, column-major, strictly lower part used) is
+// kept apart from the panel, so the row-chunk blocks of one supernode, which all read the
+// unfactored A11 from the panel, never see it half overwritten.
+struct SnDesc {
+  int64_t panel_off, l11_off;
+  int32_t ncols, ld, col_begin, rows_off;
 };
 
 __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
@@ -29,129 +34,187 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
     panel[map[q]] = vals[q];
 }
 
-// Dense LDL^T of each listed supernode's diagonal block, in LDS.
-__global__ void __launch_bounds__(256) diag_factor_kernel(const int32_t* __restrict__ list,
-                                                          SnView v, double* __restrict__ panel,
-                                                          double* __restrict__ D) {
-  __shared__ double A[kMaxCols * kMaxCols];
-  const int s = list[blockIdx.x];
-  const int n = v.ncols(s), ld = v.ld(s);
-  double* P = panel + v.panel_off[s];
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int r = idx % n, c = idx / n;
-    A[c * kMaxCols + r] = r >= c ? P[(int64_t)c * ld + r] : 0.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    const double d = A[k * kMaxCols + k];
-    const double inv = 1.0 / d;
-    const int m = n - k - 1;
-    for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-      const int i = k + 1 + idx % m, j = k + 1 + idx / m;
-      if (i >= j) A[j * kMaxCols + i] -= A[k * kMaxCols + i] * A[k * kMaxCols + j] * inv;
-    }
-    __syncthreads();
-    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) A[k * kMaxCols + i] *= inv;
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int r = idx % n, c = idx / n;
-    if (r > c) P[(int64_t)c * ld + r] = A[c * kMaxCols + r];
-    if (r == c) {
-      D[v.col_begin[s] + c] = A[c * kMaxCols + c];
-      P[(int64_t)c * ld + r] = A[c * kMaxCols + c];
-    }
-  }
-}
-
-// Below-diagonal rows: L21 = A21 L11^-T D^-1, one thread per row, items = (supernode, chunk).
-__global__ void __launch_bounds__(256) trsm_kernel(const int32_t* __restrict__ items, SnView v,
-                                                   double* __restrict__ panel,
-                                                   const double* __restrict__ D) {
-  __shared__ double G[kMaxCols * kMaxCols];  // G(k, j) = L(k, j) * d_j, j < k
-  __shared__ double Dinv[kMaxCols];
-  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
-  const int n = v.ncols(s), ld = v.ld(s), mb = ld - n;
-  double* P = panel + v.panel_off[s];
-  const double* Ds = D + v.col_begin[s];
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int k = idx % n, j = idx / n;
-    G[j * kMaxCols + k] = k > j ? P[(int64_t)j * ld + k] * Ds[j] : 0.0;
-  }
-  for (int k = threadIdx.x; k < n; k += blockDim.x) Dinv[k] = 1.0 / Ds[k];
-  __syncthreads();
-  const int i = chunk * blockDim.x + threadIdx.x;
-  if (i >= mb) return;
-  double* row = P + n + i;  // element (k) at row[k * ld]
-  for (int k = 0; k < n; ++k) {
-    double acc = row[(int64_t)k * ld];
-    for (int j = 0; j < k; ++j) acc -= row[(int64_t)j * ld] * G[j * kMaxCols + k];
-    row[(int64_t)k * ld] = acc * Dinv[k];
-  }
-}
-
-// U = L21 D L21^T tile (ti, tj), scattered as -U into ancestor panels.
-__global__ void __launch_bounds__(256) update_kernel(
-    const int32_t* __restrict__ items, SnView v, double* __restrict__ panel,
-    const double* __restrict__ D, const int32_t* __restrict__ upd_ptr,
-    const int32_t* __restrict__ upd_target, const int32_t* __restrict__ upd_seg_begin,
-    const int32_t* __restrict__ upd_rel_ptr, const int32_t* __restrict__ upd_rel) {
-  __shared__ double As[16][kTile + 1];
-  __shared__ double Bs[16][kTile + 1];
-  const int s = items[3 * blockIdx.x], ti = items[3 * blockIdx.x + 1],
-            tj = items[3 * blockIdx.x + 2];
-  const int n = v.ncols(s), ld = v.ld(s), mb = ld - n;
-  const double* P = panel + v.panel_off[s];
-  const double* Ds = D + v.col_begin[s];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4] = {};
-  for (int k0 = 0; k0 < n; k0 += 16) {
-    for (int idx = threadIdx.x; idx < 16 * kTile; idx += blockDim.x) {
-      const int kk = idx / kTile, r = idx % kTile;
-      const int k = k0 + kk;
-      const int ri = ti * kTile + r, rj = tj * kTile + r;
-      As[kk][r] = (k < n && ri < mb) ? P[(int64_t)k * ld + n + ri] : 0.0;
-      Bs[kk][r] = (k < n && rj < mb) ? P[(int64_t)k * ld + n + rj] * Ds[k] : 0.0;
-    }
-    __syncthreads();
+// Right-looking LDL^T of the 64 x 64 (padded) diagonal block held as 4 x 4 register patches:
+// thread (ty, tx) owns rows ty + 16x, cols tx + 16y. Step k = 16 kb + kk: the owners of column k
+// (tx == kk, register column kb) publish c_i = A(i, k) for i > k (zero otherwise) and
+// d = A(k, k) through LDS; every thread applies A(i, j) -= c_i c_j / d (a no-op outside the
+// trailing block because c vanishes there) and the owners replace column k by c / d. The
+// register column index kb is a compile-time constant (the kb loop is unrolled, the kk loop is
+// not), which keeps the patches in registers without unrolling all 64 steps.
+__device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*col)[kMaxCols],
+                                                 double* dval, int ty, int tx) {
 #pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll 2
     for (int kk = 0; kk < 16; ++kk) {
-      double a[4], b[4];
+      const int k = 16 * kb + kk, buf = kk & 1;
+      if (tx == kk) {
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        a[x] = As[kk][ty + 16 * x];
-        b[x] = Bs[kk][tx + 16 * x];
+        for (int x = 0; x < 4; ++x) {
+          const int i = ty + 16 * x;
+          col[buf][i] = i > k ? a[x][kb] : 0.0;
+          if (i == k) dval[buf] = a[x][kb];
+        }
       }
+      __syncthreads();
+      const double inv = 1.0 / dval[buf];
+      double ci[4], cj[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) ci[x] = col[buf][ty + 16 * x] * inv;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) cj[y] = col[buf][tx + 16 * y];
 #pragma unroll
       for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] += a[x] * b[y];
+        for (int y = 0; y < 4; ++y) a[x][y] -= ci[x] * cj[y];
+      if (tx == kk) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          if (ty + 16 * x > k) a[x][kb] = ci[x];
+      }
     }
-    __syncthreads();
   }
-  const int seg0 = upd_ptr[s], seg1 = upd_ptr[s + 1];
-  const int32_t* rows_s = v.rows + v.rows_ptr[s] + n;
+}
+
+// Fused: each block factors its supernode's diagonal block (redundantly across the row chunks of
+// that supernode; block chunk 0 writes L11 to l11 and D) and then solves L21 = A21 L11^-T D^-1
+// for its own 64-row chunk of the panel, in place.
+__global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restrict__ items,
+                                                        const SnDesc* __restrict__ desc,
+                                                        double* __restrict__ panel,
+                                                        double* __restrict__ l11,
+                                                        double* __restrict__ D) {
+  __shared__ double col[2][kMaxCols];
+  __shared__ double dval[2];
+  __shared__ double L[kMaxCols][kMaxCols + 1];  // L[j][k] = L11(j, k) for j > k, else 0
+  __shared__ double Dinv[kMaxCols];
+  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
+  const SnDesc sd = desc[s];
+  const int n = sd.ncols, ld = sd.ld, mb = ld - n;
+  double* P = panel + sd.panel_off;
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  double a[4][4], w[4][4];
+  // diagonal block (padded with identity) and this chunk of the below rows, loaded together
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int i = ty + 16 * x, j = tx + 16 * y;
+      const bool in = i < n && j < n && i >= j;
+      a[x][y] = in ? P[(int64_t)j * ld + i] : (i == j ? 1.0 : 0.0);
+      const int r = chunk * 64 + i;
+      w[x][y] = (r < mb && j < n) ? P[(int64_t)j * ld + n + r] : 0.0;
+    }
+  ldlt64_registers(a, col, dval, ty, tx);
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int i = ty + 16 * x, j = tx + 16 * y;
+      L[i][j] = i > j ? a[x][y] : 0.0;
+      if (i == j) Dinv[i] = 1.0 / a[x][y];
+    }
+  if (chunk == 0) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int i = ty + 16 * x, j = tx + 16 * y;
+        if (i < n && j < n && i > j) l11[sd.l11_off + (int64_t)j * n + i] = a[x][y];
+        if (i == j && i < n) D[sd.col_begin + i] = a[x][y];
+      }
+  }
+  __syncthreads();
+  // W L11^T = A21 column by column: column k of W is final at step k
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll 2
+    for (int kk = 0; kk < 16; ++kk) {
+      const int k = 16 * kb + kk, buf = kk & 1;
+      if (tx == kk) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) col[buf][ty + 16 * x] = w[x][kb];
+      }
+      __syncthreads();
+      double wk[4], lj[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) wk[x] = col[buf][ty + 16 * x];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) lj[y] = L[tx + 16 * y][k];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) w[x][y] -= wk[x] * lj[y];
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int r = chunk * 64 + ty + 16 * x, j = tx + 16 * y;
+      if (r < mb && j < n) P[(int64_t)j * ld + n + r] = w[x][y] * Dinv[j];
+    }
+}
+
+// U = L21 D L21^T tile (ti, tj), scattered as -U into ancestor panels. For below-row j of the
+// supernode, scat_col[j] is the panel offset of its column in the target supernode and
+// scat_rel[j] + i indexes upd_rel for the target row position of below-row i >= j.
+__global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__ items,
+                                                     const SnDesc* __restrict__ desc,
+                                                     double* __restrict__ panel,
+                                                     const double* __restrict__ D,
+                                                     const int64_t* __restrict__ scat_col,
+                                                     const int32_t* __restrict__ scat_rel,
+                                                     const int32_t* __restrict__ upd_rel) {
+  __shared__ double As[kMaxCols][kTile + 1];
+  __shared__ double Bs[kMaxCols][kTile + 1];
+  const int s = items[3 * blockIdx.x], ti = items[3 * blockIdx.x + 1],
+            tj = items[3 * blockIdx.x + 2];
+  const SnDesc sd = desc[s];
+  const int n = sd.ncols, ld = sd.ld, mb = ld - n;
+  const double* P = panel + sd.panel_off;
+  const double* Ds = D + sd.col_begin;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  // stage both row tiles (all k at once: the loads are independent)
+  for (int idx = threadIdx.x; idx < kMaxCols * kTile; idx += blockDim.x) {
+    const int k = idx / kTile, r = idx % kTile;
+    const int ri = ti * kTile + r, rj = tj * kTile + r;
+    As[k][r] = (k < n && ri < mb) ? P[(int64_t)k * ld + n + ri] : 0.0;
+    Bs[k][r] = (k < n && rj < mb) ? P[(int64_t)k * ld + n + rj] * Ds[k] : 0.0;
+  }
+  // scatter metadata for this thread's columns, fetched while the tiles load
+  int64_t cbase[4];
+  int32_t rbase[4];
 #pragma unroll
   for (int y = 0; y < 4; ++y) {
     const int j = tj * kTile + tx + 16 * y;
-    if (j >= mb) continue;
-    // segment containing below-row j
-    int lo = seg0, hi = seg1 - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (upd_seg_begin[mid] <= j) lo = mid; else hi = mid - 1;
+    const bool ok = j < mb;
+    cbase[y] = ok ? scat_col[sd.rows_off + n + j] : 0;
+    rbase[y] = ok ? scat_rel[sd.rows_off + n + j] : 0;
+  }
+  __syncthreads();
+  double acc[4][4] = {};
+#pragma unroll 16
+  for (int k = 0; k < kMaxCols; ++k) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      av[x] = As[k][ty + 16 * x];
+      bv[x] = Bs[k][tx + 16 * x];
     }
-    const int t = upd_target[lo];
-    const int jb = upd_seg_begin[lo];
-    const int col = rows_s[j] - v.col_begin[t];
-    const int ldt = v.ld(t);
-    double* Pt = panel + v.panel_off[t] + (int64_t)col * ldt;
-    const int32_t* rel = upd_rel + upd_rel_ptr[lo];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc[x][y] += av[x] * bv[y];
+  }
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    const int j = tj * kTile + tx + 16 * y;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
       const int i = ti * kTile + ty + 16 * x;
-      if (i >= mb || i < j) continue;
-      unsafeAtomicAdd(Pt + rel[i - jb], -acc[x][y]);
+      if (j < mb && i < mb && i >= j)
+        unsafeAtomicAdd(panel + cbase[y] + upd_rel[rbase[y] + i], -acc[x][y]);
     }
   }
 }
@@ -171,53 +234,96 @@ __global__ void dscale_kernel(int32_t n, const double* __restrict__ D, double* _
   if (i < n) y[i] /= D[i];
 }
 
-// Forward: y_own = L11^-1 y_own; y_below -= L21 y_own (atomics: siblings share ancestors).
-__global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ list, SnView v,
-                                                  const double* __restrict__ panel,
-                                                  double* __restrict__ y) {
-  __shared__ double ys[kMaxCols];
-  const int s = list[blockIdx.x];
-  const int n = v.ncols(s), ld = v.ld(s), mb = ld - n;
-  const double* P = panel + v.panel_off[s];
-  double* yo = y + v.col_begin[s];
-  for (int k = threadIdx.x; k < n; k += blockDim.x) ys[k] = yo[k];
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) ys[i] -= P[(int64_t)k * ld + i] * ys[k];
-    __syncthreads();
-  }
-  for (int k = threadIdx.x; k < n; k += blockDim.x) yo[k] = ys[k];
-  const int32_t* rows = v.rows + v.rows_ptr[s] + n;
-  for (int i = threadIdx.x; i < mb; i += blockDim.x) {
-    double acc = 0;
-    for (int k = 0; k < n; ++k) acc += P[(int64_t)k * ld + n + i] * ys[k];
-    unsafeAtomicAdd(&y[rows[i]], -acc);
+// Stages the strictly lower part of L11 (n x n column-major at Lc, padded with zeros; unit
+// diagonal implied) into LDS: Ls[i][k].
+__device__ __forceinline__ void stage_l11(const double* Lc, int n, double (*Ls)[kMaxCols + 1]) {
+  for (int idx = threadIdx.x; idx < kMaxCols * kMaxCols; idx += blockDim.x) {
+    const int i = idx % kMaxCols, k = idx / kMaxCols;
+    Ls[i][k] = (i < n && k < n && i > k) ? Lc[(int64_t)k * n + i] : 0.0;
   }
 }
 
-// Backward: y_own -= L21^T y_below; y_own = L11^-T y_own.
-__global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ list, SnView v,
+// Forward: y_own = L11^-1 y_own (one wavefront, lane i holds y_i, L11 from LDS);
+// y_below -= L21 y_own with atomics (siblings at the same level share ancestors).
+__global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ list,
+                                                  const SnDesc* __restrict__ desc,
+                                                  const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
+                                                  const double* __restrict__ l11,
                                                   double* __restrict__ y) {
+  __shared__ double Ls[kMaxCols][kMaxCols + 1];
   __shared__ double ys[kMaxCols];
   const int s = list[blockIdx.x];
-  const int n = v.ncols(s), ld = v.ld(s), mb = ld - n;
-  const double* P = panel + v.panel_off[s];
-  double* yo = y + v.col_begin[s];
-  const int32_t* rows = v.rows + v.rows_ptr[s] + n;
+  const SnDesc sd = desc[s];
+  const int n = sd.ncols, ld = sd.ld, mb = ld - n;
+  const double* P = panel + sd.panel_off;
+  double* yo = y + sd.col_begin;
+  stage_l11(l11 + sd.l11_off, n, Ls);
+  double yi = 0;
+  if (threadIdx.x < kMaxCols && threadIdx.x < n) yi = yo[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) {
+      const double yk = __shfl(yi, k, 64);
+      yi -= Ls[i][k] * yk;
+    }
+    if (i < n) yo[i] = yi;
+    ys[i] = i < n ? yi : 0.0;
+  }
+  __syncthreads();
+  const int32_t* rw = rows + sd.rows_off + n;
+  for (int i = threadIdx.x; i < mb; i += blockDim.x) {
+    double acc = 0;
+#pragma unroll 16
+    for (int k = 0; k < kMaxCols; ++k)
+      if (k < n) acc += P[(int64_t)k * ld + n + i] * ys[k];
+    unsafeAtomicAdd(&y[rw[i]], -acc);
+  }
+}
+
+// Backward: y_own -= L21^T y_below (waves split the columns, lanes the rows); then
+// y_own = L11^-T y_own by one wavefront from LDS.
+__global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ list,
+                                                  const SnDesc* __restrict__ desc,
+                                                  const int32_t* __restrict__ rows,
+                                                  const double* __restrict__ panel,
+                                                  const double* __restrict__ l11,
+                                                  double* __restrict__ y) {
+  __shared__ double Ls[kMaxCols][kMaxCols + 1];
+  __shared__ double ys[kMaxCols];
+  __shared__ double yb[1024];
+  const int s = list[blockIdx.x];
+  const SnDesc sd = desc[s];
+  const int n = sd.ncols, ld = sd.ld, mb = ld - n;
+  const double* P = panel + sd.panel_off;
+  double* yo = y + sd.col_begin;
+  const int32_t* rw = rows + sd.rows_off + n;
+  stage_l11(l11 + sd.l11_off, n, Ls);
+  const bool cached = mb <= 1024;
+  if (cached)
+    for (int i = threadIdx.x; i < mb; i += blockDim.x) yb[i] = y[rw[i]];
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int k = wave; k < n; k += nw) {
     double acc = 0;
-    for (int i = lane; i < mb; i += 64) acc += P[(int64_t)k * ld + n + i] * y[rows[i]];
+    for (int i = lane; i < mb; i += 64)
+      acc += P[(int64_t)k * ld + n + i] * (cached ? yb[i] : y[rw[i]]);
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) ys[k] = yo[k] - acc;
   }
   __syncthreads();
-  for (int k = n - 1; k >= 0; --k) {
-    for (int i = threadIdx.x; i < k; i += blockDim.x) ys[i] -= P[(int64_t)i * ld + k] * ys[k];
-    __syncthreads();
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+    double yi = i < n ? ys[i] : 0.0;
+#pragma unroll
+    for (int k = kMaxCols - 1; k >= 0; --k) {
+      const double yk = __shfl(yi, k, 64);
+      yi -= Ls[k][i] * yk;
+    }
+    if (i < n) yo[i] = yi;
   }
-  for (int k = threadIdx.x; k < n; k += blockDim.x) yo[k] = ys[k];
 }
 
 }  // namespace
@@ -231,19 +337,46 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   panel_.resize(std::max<int64_t>(1, sym.panel_size()));
   D_.resize(std::max(1, sym.n));
   y_.resize(std::max(1, sym.n));
-  panel_off_.upload(sym.sn_panel_off, s);
-  col_begin_.upload(sym.sn_col_begin, s);
-  rows_ptr_.upload(sym.sn_rows_ptr, s);
   rows_.upload(sym.sn_rows, s);
   perm_.upload(sym.perm, s);
-  upd_ptr_.upload(sym.upd_ptr, s);
-  upd_target_.upload(sym.upd_target, s);
-  upd_seg_begin_.upload(sym.upd_seg_begin, s);
-  upd_rel_ptr_.upload(sym.upd_rel_ptr, s);
   upd_rel_.upload(sym.upd_rel, s);
-  upd_col_off_.upload(sym.upd_col_off, s);
+  // supernode descriptors
+  std::vector<SnDesc> desc(std::max(1, sym.n_snodes));
+  int64_t l11_size = 0;
+  for (int k = 0; k < sym.n_snodes; ++k) {
+    const int n = sym.sn_col_begin[k + 1] - sym.sn_col_begin[k];
+    desc[k] = SnDesc{sym.sn_panel_off[k], l11_size, n, sym.sn_rows_ptr[k + 1] - sym.sn_rows_ptr[k],
+                     sym.sn_col_begin[k], sym.sn_rows_ptr[k]};
+    l11_size += (int64_t)n * n;
+  }
+  l11_.resize(std::max<int64_t>(1, l11_size));
+  fill_zero(l11_.get(), l11_.size(), s);
+  desc_.resize(desc.size() * sizeof(SnDesc));
+  HIP_OK(hipMemcpyAsync(desc_.get(), desc.data(), desc.size() * sizeof(SnDesc),
+                        hipMemcpyHostToDevice, s));
+  // scatter targets per below-row
+  std::vector<int64_t> scol(std::max<size_t>(1, sym.sn_rows.size()), 0);
+  std::vector<int32_t> srel(std::max<size_t>(1, sym.sn_rows.size()), 0);
+  for (int k = 0; k < sym.n_snodes; ++k) {
+    const int n = sym.sn_col_begin[k + 1] - sym.sn_col_begin[k];
+    const int q0 = sym.sn_rows_ptr[k] + n;
+    for (int seg = sym.upd_ptr[k]; seg < sym.upd_ptr[k + 1]; ++seg) {
+      const int t = sym.upd_target[seg];
+      const int jb = sym.upd_seg_begin[seg];
+      const int je = seg + 1 < sym.upd_ptr[k + 1] ? sym.upd_seg_begin[seg + 1]
+                                                  : sym.sn_rows_ptr[k + 1] - q0;
+      const int ldt = sym.sn_rows_ptr[t + 1] - sym.sn_rows_ptr[t];
+      for (int j = jb; j < je; ++j) {
+        const int col = sym.sn_rows[q0 + j] - sym.sn_col_begin[t];
+        scol[q0 + j] = sym.sn_panel_off[t] + (int64_t)col * ldt;
+        srel[q0 + j] = sym.upd_rel_ptr[seg] - jb;
+      }
+    }
+  }
+  scat_col_.upload(scol, s);
+  scat_rel_.upload(srel, s);
   // Work lists per level
-  std::vector<int32_t> snl, trsm, upd;
+  std::vector<int32_t> snl, dt, upd;
   lvl_snode_ptr_.assign(1, 0);
   lvl_trsm_ptr_.assign(1, 0);
   lvl_upd_ptr_.assign(1, 0);
@@ -253,9 +386,10 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
       snl.push_back(sn);
       const int n = sym.sn_col_begin[sn + 1] - sym.sn_col_begin[sn];
       const int mb = sym.sn_rows_ptr[sn + 1] - sym.sn_rows_ptr[sn] - n;
-      for (int c = 0; c * 256 < mb; ++c) {
-        trsm.push_back(sn);
-        trsm.push_back(c);
+      const int chunks = std::max(1, (mb + 63) / 64);
+      for (int c = 0; c < chunks; ++c) {
+        dt.push_back(sn);
+        dt.push_back(c);
       }
       const int T = (mb + kTile - 1) / kTile;
       for (int ti = 0; ti < T; ++ti)
@@ -266,11 +400,11 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
         }
     }
     lvl_snode_ptr_.push_back((int32_t)snl.size());
-    lvl_trsm_ptr_.push_back((int32_t)trsm.size() / 2);
+    lvl_trsm_ptr_.push_back((int32_t)dt.size() / 2);
     lvl_upd_ptr_.push_back((int32_t)upd.size() / 3);
   }
   lvl_snodes_.upload(snl, s);
-  trsm_items_.upload(trsm, s);
+  trsm_items_.upload(dt, s);
   upd_items_.upload(upd, s);
   HIP_OK(hipStreamSynchronize(s));
 }
@@ -287,21 +421,18 @@ void SupernodalLDLT::factorize(const double* a_values, hipStream_t s) {
 }
 
 void SupernodalLDLT::factorize_assembled(hipStream_t s) {
-  const SnView v{panel_off_.get(), col_begin_.get(), rows_ptr_.get(), rows_.get()};
+  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
-    const int ns = lvl_snode_ptr_[l + 1] - lvl_snode_ptr_[l];
-    hipLaunchKernelGGL(diag_factor_kernel, dim3(ns), dim3(256), 0, s,
-                       lvl_snodes_.get() + lvl_snode_ptr_[l], v, panel_.get(), D_.get());
     const int nt = lvl_trsm_ptr_[l + 1] - lvl_trsm_ptr_[l];
     if (nt)
-      hipLaunchKernelGGL(trsm_kernel, dim3(nt), dim3(256), 0, s,
-                         trsm_items_.get() + 2 * lvl_trsm_ptr_[l], v, panel_.get(), D_.get());
+      hipLaunchKernelGGL(diag_trsm_kernel, dim3(nt), dim3(256), 0, s,
+                         trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(), l11_.get(),
+                         D_.get());
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
     if (nu)
       hipLaunchKernelGGL(update_kernel, dim3(nu), dim3(256), 0, s,
-                         upd_items_.get() + 3 * lvl_upd_ptr_[l], v, panel_.get(), D_.get(),
-                         upd_ptr_.get(), upd_target_.get(), upd_seg_begin_.get(),
-                         upd_rel_ptr_.get(), upd_rel_.get());
+                         upd_items_.get() + 3 * lvl_upd_ptr_[l], desc, panel_.get(), D_.get(),
+                         scat_col_.get(), scat_rel_.get(), upd_rel_.get());
   }
   HIP_OK(hipGetLastError());
 }
@@ -309,19 +440,21 @@ void SupernodalLDLT::factorize_assembled(hipStream_t s) {
 void SupernodalLDLT::solve(const double* b, double* x, hipStream_t s) {
   const int n = sym_.n;
   if (n == 0) return;
-  const SnView v{panel_off_.get(), col_begin_.get(), rows_ptr_.get(), rows_.get()};
+  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(), b,
                      y_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
     const int ns = lvl_snode_ptr_[l + 1] - lvl_snode_ptr_[l];
     hipLaunchKernelGGL(fwd_kernel, dim3(ns), dim3(256), 0, s,
-                       lvl_snodes_.get() + lvl_snode_ptr_[l], v, panel_.get(), y_.get());
+                       lvl_snodes_.get() + lvl_snode_ptr_[l], desc, rows_.get(), panel_.get(),
+                       l11_.get(), y_.get());
   }
   hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(), y_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {
     const int ns = lvl_snode_ptr_[l + 1] - lvl_snode_ptr_[l];
     hipLaunchKernelGGL(bwd_kernel, dim3(ns), dim3(256), 0, s,
-                       lvl_snodes_.get() + lvl_snode_ptr_[l], v, panel_.get(), y_.get());
+                       lvl_snodes_.get() + lvl_snode_ptr_[l], desc, rows_.get(), panel_.get(),
+                       l11_.get(), y_.get());
   }
   hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(),
                      y_.get(), x);

@@ -151,12 +151,15 @@ __global__ void __launch_bounds__(256) lde_partial_kernel(const double* __restri
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
-__global__ void sum_into_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double acc = 0;
-    for (int i = 0; i < nb; ++i) acc += part[i];
-    *out = acc;
-  }
+__global__ void __launch_bounds__(256) sum_into_kernel(const double* __restrict__ part, int nb,
+                                                       double* __restrict__ out) {
+  __shared__ double ws[4];
+  double acc = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += part[i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // ---- CSC backend kernels ----
@@ -460,7 +463,7 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
     const int nb = std::max(1, std::min(kReduceBlocks, grid_for(N, 256)));
     hipLaunchKernelGGL(lde_partial_kernel, dim3(nb), dim3(256), 0, stream_, update_.get(),
                        backend_->rhs(init_), backend_->damping(), (int64_t)N, partials_.get());
-    hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(1), 0, stream_, partials_.get(), nb,
+    hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, stream_, partials_.get(), nb,
                        &dstate_->lde);
   }
   if (p_.enable_bold_updates && have_last_update_) {

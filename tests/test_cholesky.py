@@ -95,14 +95,22 @@ def test_solve_matches_oracle_and_dense(name, ordering):
                            None if ordering == capi.ORDERING_NATURAL else ch.inv_permutation())
     oc.factorize(vals)
     xo = oc.solve(b)
-    A = ow.csc_to_dense(n, cp, ri, vals)
-    A = A + np.tril(A, -1).T
-    scale = np.linalg.norm(xo)
-    assert np.linalg.norm(x - xo) <= 1e-9 * scale * max(1.0, np.linalg.cond(A) * 1e-6)
-    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * max(1.0, np.linalg.norm(A, 2) * np.linalg.norm(x) / np.linalg.norm(b))
+    import scipy.sparse as sp
+    L = sp.csc_matrix((vals, ri, cp), shape=(n, n))
+    A = L + sp.tril(L, -1).T
+    # Forward error of two backward-stable LDL^T solves differs by up to ~cond(A) * eps: the
+    # damped Hessians are well conditioned (1e-9 holds), the reference's random M^T M recipe is
+    # not (random_spd_300 has cond 5.5e9, where the oracle and LAPACK already differ by 3e-8).
+    cond = np.linalg.cond(A.toarray()) if n <= 1000 else 1.0
+    rtol = max(1e-9, 20 * cond * np.finfo(float).eps)
+    assert np.linalg.norm(x - xo) <= rtol * np.linalg.norm(xo)
+    # backward error (independent of conditioning)
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * abs(A).sum(axis=0).max() * np.linalg.norm(x)
     # D matches the oracle's D in the same permuted order
-    np.testing.assert_allclose(ch.d(), oc.factor()["d"], rtol=1e-9, atol=1e-12 * np.abs(oc.factor()["d"]).max())
+    d_ref = oc.factor()["d"]
+    np.testing.assert_allclose(ch.d(), d_ref, rtol=rtol, atol=1e-12 * np.abs(d_ref).max())
     # matrix right-hand side
     B = rng.standard_normal((n, 3))
     X = ch.solve(B)
-    np.testing.assert_allclose(X, oc.solve(B), rtol=0, atol=1e-9 * np.abs(oc.solve(B)).max() * max(1.0, np.linalg.cond(A) * 1e-6))
+    XO = oc.solve(B)
+    assert np.linalg.norm(X - XO) <= rtol * np.linalg.norm(XO)
