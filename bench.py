@@ -68,7 +68,7 @@ def run_gpu(args):
     prob = capi.gen_ba_snavely(1000, 200000, 42)
     t_gen = time.time() - t0
     opts = capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
-                               ordering=capi.ORDERING_MIN_DEGREE)
+                               ordering=capi.ORDERING_NESTED_DISSECTION)
     t0 = time.time()
     opt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0), opts)
     t_setup = time.time() - t0
@@ -91,12 +91,33 @@ def run_gpu(args):
     target = None
     if ORACLE_FIXTURE.exists():
         target = json.loads(ORACLE_FIXTURE.read_text())["final_cost"]
-    plain = None
+    plain = fast = None
     if not args.no_converge:
         plain = time_to_cost(prob, opts, target)
+        fast = time_to_cost(prob, fast_options(args), target)
+    # the fast form's iteration rate and phases over the same K steps
+    fopt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0),
+                             fast_options(args))
+    fopt.optimize_resident(num_iterations=args.warmup)
+    t0 = time.perf_counter()
+    fres = fopt.optimize_resident(num_iterations=args.steps)
+    fast_ms = 1e3 * (time.perf_counter() - t0) / (len(fres["iterations"]) - 1)
+    fopt.enable_timing(True)
+    fopt.optimize_resident(num_iterations=args.steps)
+    fast_phases = fopt.phase_times()
     return dict(prob=prob, opt=opt, info=info, t_gen=t_gen, t_setup=t_setup,
                 ms_per_step=ms_per_step, n_iter=n_iter, phases=phases, res=res, plain=plain,
-                target=target)
+                fast=fast, target=target, fast_ms=fast_ms, fast_phases=fast_phases,
+                fast_lin=fopt.linear_solver_stats())
+
+
+def fast_options(args):
+    """The fast form: block-Jacobi PCG on the reduced camera system (FP64), falling back to the
+    supernodal LDL^T when CG does not reach the tolerance."""
+    return capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
+                               ordering=capi.ORDERING_NESTED_DISSECTION,
+                               use_pcg=1, pcg_max_iters=args.pcg_max_iters,
+                               pcg_rel_tol=args.pcg_tol)
 
 
 def time_to_cost(prob, opts, target):
@@ -107,6 +128,7 @@ def time_to_cost(prob, opts, target):
     cres = conv.optimize_resident()
     t_conv = time.perf_counter() - t0
     times = conv.iteration_times()
+    lin = conv.linear_solver_stats()
     best = np.minimum.accumulate([it["new_error"] for it in cres["iterations"]])
     final_cost = float(best[-1])
     ttc = None
@@ -115,8 +137,14 @@ def time_to_cost(prob, opts, target):
         ttc = float(times[hit[0]] / 1e3) if hit.size else None
     return dict(time_to_cost_s=ttc, final_cost=final_cost,
                 iterations=len(cres["iterations"]) - 1, converge_s=t_conv,
-                status=cres["status"],
+                status=cres["status"], linear_solver=lin,
                 rel_gap=None if target is None else (final_cost - target) / target)
+
+
+def speedup(plain, fast):
+    if not plain or not fast or not plain["time_to_cost_s"] or not fast["time_to_cost_s"]:
+        return None
+    return plain["time_to_cost_s"] / fast["time_to_cost_s"]
 
 
 def roofline(r):
@@ -140,7 +168,36 @@ def roofline(r):
                 algorithmic_bytes=model[best])
 
 
-def cpu_baseline(prob, iters=2):
+def batch_leg(n_problems=4096, cpu_sample=64):
+    """Batched form: n independent robot-3D-localization problems (5-50 poses, 5-40 landmarks,
+    30 % visible) in one persistent launch, FP64 and FP32-Jacobian forms; the CPU oracle solves
+    a sample of the same problems one after another on one core."""
+    probs = capi.gen_localization_batch(n_problems, seed=42)
+    params = capi.default_params(iterations=50)
+    out = {"workload": f"robot3d_localization_x{n_problems}", "n_problems": n_problems}
+    for name, prec in (("f64", capi.PRECISION_F64), ("f32_jac", capi.PRECISION_F32_JAC)):
+        b = capi.HipBatch(probs, params, precision=prec)
+        b.solve(records=False)  # warm-up
+        runs = [b.solve(records=False) for _ in range(3)]
+        r = min(runs, key=lambda x: x["total_ms"])
+        ok = sum(p["status"] == capi.STATUS_SUCCESS for p in r["problems"])
+        out[name] = {"problems_per_s": n_problems / (r["total_ms"] * 1e-3),
+                     "launch_ms": r["total_ms"],
+                     "p50_solve_ms": float(np.percentile(r["solve_ms"], 50)),
+                     "p99_solve_ms": float(np.percentile(r["solve_ms"], 99)),
+                     "converged": ok}
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_wrap as ow
+    t0 = time.perf_counter()
+    for prob in probs[:cpu_sample]:
+        ow.OracleProblem(prob).optimize(params)
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"problems_per_s": cpu_sample / dt, "cores": 1, "kind": "port",
+                           "sample": f"first {cpu_sample} problems, FP64 oracle LM, sequential"}
+    return out
+
+
+def cpu_baseline(prob, iters=1):
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
 
@@ -149,12 +206,11 @@ def cpu_baseline(prob, iters=2):
     t_create = time.time() - t0
     secs, _ = ow.time_iterations(op, lm_params(iters, early_exit=0.0), iters, solver=ow.SCHUR,
                                  c_dim=int(prob.info["c_dim"]))
-    steady = float(np.mean(secs[1:])) if len(secs) > 1 else float(secs[0])
-    return dict(value=1.0 / steady, unit="LM iters/s", cores=1, kind="port",
+    per_iter = float(np.mean(secs))
+    return dict(value=1.0 / per_iter, unit="LM iters/s", cores=1, kind="port",
                 sample=f"CPU FP64 oracle (restated reference LM + SparseSchurSolver), same 1k-cam "
-                       f"problem, {iters} LM iterations after {t_create:.1f} s setup; first "
-                       f"iteration (incl. symbolic analysis) {secs[0]:.2f} s, steady "
-                       f"{steady:.2f} s/iter")
+                       f"problem, {iters} LM iteration(s) after {t_create:.1f} s setup: "
+                       f"{per_iter:.2f} s/iter (incl. the one-time symbolic analysis)")
 
 
 def main():
@@ -163,6 +219,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
+    ap.add_argument("--pcg-tol", type=float, default=1e-3)
+    ap.add_argument("--pcg-max-iters", type=int, default=200)
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the run-to-convergence (time-to-cost) leg, e.g. under a profiler")
     args = ap.parse_args()
@@ -190,13 +249,23 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded generator, BA along a random-walk trajectory)",
         "config": {"workload": "ba_snavely_1000cams_200kpts_1Mobs_huber_schur",
-                   "n_obs": int(r["prob"].info["n_obs"]), "solver": "schur+supernodal_ldlt",
+                   "n_obs": int(r["prob"].info["n_obs"]), "solver": "schur+supernodal_ldlt", "ordering": "nested_dissection",
                    "lambda": "dynamic", "form": "plain_fp64"},
-        "time_to_cost": {"target_cost": r["target"], "plain": r["plain"], "fast": None},
+        "time_to_cost": {"target_cost": r["target"], "plain": r["plain"], "fast": r["fast"],
+                         "fast_form": {"solver": "schur+block_jacobi_pcg", "dtype": "f64",
+                                       "pcg_rel_tol": args.pcg_tol,
+                                       "pcg_max_iters": args.pcg_max_iters},
+                         "speedup": speedup(r["plain"], r["fast"])},
         "setup_s": r["t_setup"],
         "phases_ms_per_iter": {k: v[0] / max(1, v[1]) for k, v in r["phases"].items() if v[1]},
+        "fast_ms_per_step": r["fast_ms"],
+        "fast_phases_ms_per_iter": {k: v[0] / max(1, v[1]) for k, v in r["fast_phases"].items()
+                                    if v[1]},
+        "fast_linear_solver": r["fast_lin"],
         "roofline": roof,
     }
+    if not args.no_batch:
+        out["batch"] = batch_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(r["prob"])
     print(json.dumps(out))

@@ -51,6 +51,30 @@ int symhip_linearize(symhip_problem_t problem, const double* values, double* res
 
 
 /* ---------------------------------------------------------------------------------------------
+ * Batched optimizer: many small independent problems in one persistent launch (one wavefront per
+ * problem runs the whole LM loop; no reference counterpart, the reference runs one Optimizer per
+ * thread, opt/optimizer.h:23). Supported problems: robot-3D-localization structure (optimized
+ * keys are up to 64 Pose3, MatchingFactor on one pose, OdometryFactor between consecutive
+ * optimized poses). The LM semantics are those of symhip_optimizer_optimize.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct symhip_batch* symhip_batch_t;
+
+/* precision: SYMHIP_PRECISION_F64, or SYMHIP_PRECISION_F32_JAC (factor bodies in FP32; Values,
+   Hessian blocks, solve and cost in FP64). */
+int symhip_batch_create(const symhip_problem_desc* descs, int32_t n_problems,
+                        const symhip_optimizer_params* params, int32_t precision, double epsilon,
+                        symhip_batch_t* out);
+int symhip_batch_destroy(symhip_batch_t batch);
+/* Total Values storage (doubles) over all problems, in problem order. */
+int64_t symhip_batch_storage_size(symhip_batch_t batch);
+/* Solves every problem from its initial Values. Outputs may be NULL: values_out (storage size)
+   gets the best Values; stats_out one record per problem; iters_out (params.iterations + 1) per
+   problem; solve_ms the per-problem time inside the launch; total_ms the launch time. */
+int symhip_batch_solve(symhip_batch_t batch, double* values_out,
+                       symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
+                       double* solve_ms, double* total_ms);
+
+/* ---------------------------------------------------------------------------------------------
  * Optimizer: sym::Optimizer<double> + LevenbergMarquardtSolver with the linearization, linear
  * solve, damping, retraction and accept/lambda logic on the device (opt/optimizer.tcc:21-89,
  * opt/levenberg_marquardt_solver.tcc:138-343).
@@ -59,7 +83,11 @@ enum symhip_linear_solver {
   SYMHIP_SOLVER_CHOLESKY = 0, /* SparseCholeskySolver replacement: supernodal LDL^T on H */
   SYMHIP_SOLVER_SCHUR = 1,    /* SparseSchurSolver replacement: landmark elimination + RCS */
 };
-enum symhip_ordering { SYMHIP_ORDERING_NATURAL = 0, SYMHIP_ORDERING_MIN_DEGREE = 1 };
+enum symhip_ordering {
+  SYMHIP_ORDERING_NATURAL = 0,
+  SYMHIP_ORDERING_MIN_DEGREE = 1,
+  SYMHIP_ORDERING_NESTED_DISSECTION = 2, /* recursive BFS-level separators (METIS's family) */
+};
 enum symhip_precision {
   SYMHIP_PRECISION_F64 = 0,      /* plain FP64 everywhere */
   SYMHIP_PRECISION_F32_JAC = 1,  /* FP32 Jacobians/Hessian blocks, FP64 residual/cost/decisions */
@@ -69,10 +97,21 @@ typedef struct symhip_solver_options {
   int32_t linear_solver; /* symhip_linear_solver */
   int32_t ordering;      /* symhip_ordering (fill-reducing ordering of the factorized matrix) */
   int32_t precision;     /* symhip_precision */
-  int32_t use_pcg;       /* Schur path: preconditioned CG on the reduced camera system */
+  int32_t use_pcg;       /* Schur path: block-Jacobi preconditioned CG on the reduced camera
+                            system. 0 off (direct LDL^T), 1 truncated (an iterate that missed
+                            pcg_rel_tol within pcg_max_iters is used; LDL^T only when CG
+                            stalls), 2 strict (LDL^T whenever the tolerance is missed) */
   int32_t pcg_max_iters;
   double pcg_rel_tol;
+  int32_t pcg_preconditioner; /* SYMHIP_PRECOND_BLOCK_JACOBI or SYMHIP_PRECOND_LEARNED */
 } symhip_solver_options;
+
+enum symhip_pcg_preconditioner {
+  SYMHIP_PRECOND_BLOCK_JACOBI = 0, /* inverse diagonal camera blocks */
+  SYMHIP_PRECOND_LEARNED = 1,      /* block Jacobi corrected per camera by the trained network
+                                      (symhip_optimizer_train_preconditioner); untrained it is
+                                      exactly block Jacobi */
+};
 
 typedef struct symhip_optimizer* symhip_optimizer_t;
 
@@ -126,6 +165,40 @@ int symhip_optimizer_phase_times(symhip_optimizer_t opt, double* total_ms, int64
    out[2]=panel doubles, out[3]=levels, out[4]=factor flops (rounded), out[5]=reduced-camera-
    system blocks (Schur path, else 0), out[6]=landmark observation pairs, out[7]=observations. */
 int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out8);
+/* Iterative (PCG) linear solver statistics since creation: out[0]=solves, out[1]=total CG
+   iterations, out[2]=fallbacks to the direct LDL^T, out[3]=most iterations in one solve.
+   No reference counterpart (the reference solvers are direct). */
+int symhip_optimizer_linear_solver_stats(symhip_optimizer_t opt, int64_t* out4);
+
+/* Learned PCG preconditioner (Schur path, use_pcg != 0, 9-dof cameras). No reference
+   counterpart. The network has SYMHIP_LEARNED_PRECOND_PARAMS float parameters. */
+#define SYMHIP_LEARNED_PRECOND_PARAMS 21744
+/* Keep copies of the next max reduced camera systems and right-hand sides as the optimizer
+   assembles them (training data); clears earlier copies. */
+int symhip_optimizer_log_systems(symhip_optimizer_t opt, int32_t max);
+int32_t symhip_optimizer_logged_systems(symhip_optimizer_t opt);
+/* Block pattern of the reduced camera system (full symmetric block-CSR, 9x9 row-major blocks):
+   returns the camera count n (< 0 on error); row_ptr (n + 1) and col (row_ptr[n]) may be NULL. */
+int32_t symhip_optimizer_rcs_pattern(symhip_optimizer_t opt, int32_t* row_ptr, int32_t* col);
+int symhip_optimizer_logged_system(symhip_optimizer_t opt, int32_t idx, double* blocks,
+                                   double* rhs);
+/* get / set may be NULL. */
+int symhip_optimizer_preconditioner_params(symhip_optimizer_t opt, float* get, const float* set);
+/* Loss and gradient on logged system idx for host probes X (P columns of n * 9, P <= 7). */
+int symhip_optimizer_preconditioner_loss_grad(symhip_optimizer_t opt, int32_t idx,
+                                              const double* X, int32_t P, float* loss,
+                                              float* grad);
+/* Adam training on logged systems 0 .. n_systems - 1 (all when n_systems <= 0), round robin,
+   fresh probes each step; losses (steps) may be NULL. */
+int symhip_optimizer_train_preconditioner(symhip_optimizer_t opt, int32_t steps, float lr,
+                                          int32_t P, uint64_t seed, int32_t n_systems,
+                                          float* losses);
+/* PCG on logged system idx with block Jacobi (learned = 0) or the network (1): iterations to
+   rel_tol (negative when max_iters was reached) and the relative error of the step against the
+   supernodal LDL^T solve (step_err may be NULL). */
+int symhip_optimizer_eval_preconditioner(symhip_optimizer_t opt, int32_t idx, double rel_tol,
+                                         int32_t max_iters, int32_t learned, int32_t* iterations,
+                                         double* step_err);
 
 /* ---------------------------------------------------------------------------------------------
  * Linear solver adapter: SparseCholeskySolver (opt/sparse_cholesky/sparse_cholesky_solver.h:

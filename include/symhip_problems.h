@@ -43,6 +43,14 @@ symhip_generated* symhip_gen_pose_graph(int32_t n_nodes, int32_t n_loops, uint32
 symhip_generated* symhip_gen_localization(int32_t n_poses, int32_t n_landmarks,
                                           double visible_fraction, uint32_t seed);
 
+/* Reads a Bundle Adjustment in the Large text file, as the reference example's ReadProblem
+   (examples/bundle_adjustment_in_the_large/bundle_adjustment_in_the_large.cc:60-118): Pose3
+   from angle-axis via Rot3::FromTangent, keys 'c', 'i', 'p', 'P', 'e'. huber_k > 0 uses the
+   Huber-whitened Snavely factor. NULL on a malformed or missing file. */
+symhip_generated* symhip_load_bal(const char* path, double huber_k);
+/* Writes the Snavely observations/cameras/points of a problem as BAL text (0 on success). */
+int symhip_save_bal(const symhip_problem_desc* desc, const char* path);
+
 void symhip_gen_free(symhip_generated* g);
 /* Points `out` at the generator's arrays (valid until symhip_gen_free). */
 int symhip_gen_desc(const symhip_generated* g, symhip_problem_desc* out);

@@ -44,6 +44,7 @@ I32P = C.POINTER(C.c_int32)
 I64P = C.POINTER(C.c_int64)
 I8P = C.POINTER(C.c_int8)
 F64P = C.POINTER(C.c_double)
+F32P = C.POINTER(C.c_float)
 
 
 class ProblemDesc(C.Structure):
@@ -212,6 +213,7 @@ def problems_lib():
             ("symhip_gen_rot_frozen", [C.c_int32, C.c_uint32]),
             ("symhip_gen_ba_circle", [C.c_int32, C.c_int32, C.c_uint32]),
             ("symhip_gen_ba_snavely", [C.c_int32, C.c_int32, C.c_uint32]),
+            ("symhip_load_bal", [C.c_char_p, C.c_double]),
             ("symhip_gen_pose_graph", [C.c_int32, C.c_int32, C.c_uint32]),
             ("symhip_gen_localization", [C.c_int32, C.c_int32, C.c_double, C.c_uint32]),
         ]:
@@ -224,6 +226,8 @@ def problems_lib():
         lib.symhip_gen_info.restype = C.c_double
         lib.symhip_gen_storage_size.argtypes = [vp]
         lib.symhip_gen_storage_size.restype = C.c_int64
+        lib.symhip_save_bal.argtypes = [C.POINTER(ProblemDesc), C.c_char_p]
+        lib.symhip_save_bal.restype = C.c_int
         _problems_lib = lib
     return _problems_lib
 
@@ -269,6 +273,22 @@ def gen_rot_frozen(num_keys=3, seed=42):
 
 def gen_ba_circle(n_cams=20, n_points=1000, seed=42):
     return _from_generated(problems_lib().symhip_gen_ba_circle(n_cams, n_points, seed))
+
+
+def load_bal(path, huber_k=0.0):
+    """Bundle Adjustment in the Large text file -> Problem (the reference example's
+    ReadProblem, bundle_adjustment_in_the_large.cc:60-118)."""
+    h = problems_lib().symhip_load_bal(str(path).encode(), float(huber_k))
+    if not h:
+        raise RuntimeError(f"could not read BAL problem from {path}")
+    return _from_generated(h)
+
+
+def save_bal(prob, path):
+    """Writes the Snavely part of a problem as BAL text."""
+    d = prob.desc()
+    if problems_lib().symhip_save_bal(C.byref(d), str(path).encode()) != 0:
+        raise RuntimeError(f"could not write BAL problem to {path}")
 
 
 def gen_ba_snavely(n_cams=1000, n_points=200000, seed=42):
@@ -322,6 +342,7 @@ class SolverOptions(C.Structure):
         ("use_pcg", C.c_int32),
         ("pcg_max_iters", C.c_int32),
         ("pcg_rel_tol", C.c_double),
+        ("pcg_preconditioner", C.c_int32),
     ]
 
 
@@ -329,8 +350,10 @@ SOLVER_CHOLESKY, SOLVER_SCHUR = 0, 1
 PHASE_NAMES = ["linearize", "damp", "point_elim", "rcs_assemble", "reduced_rhs", "factor",
                "solve", "back_sub", "retract", "decide", "lin_points", "lin_cams"]
 NUM_PHASES = len(PHASE_NAMES)
-ORDERING_NATURAL, ORDERING_MIN_DEGREE = 0, 1
+ORDERING_NATURAL, ORDERING_MIN_DEGREE, ORDERING_NESTED_DISSECTION = 0, 1, 2
 PRECISION_F64, PRECISION_F32_JAC = 0, 1
+PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
+LEARNED_PRECOND_PARAMS = 21744
 
 _vp = C.c_void_p
 _EXTRA_SIGNATURES: list = [
@@ -346,6 +369,24 @@ _EXTRA_SIGNATURES: list = [
                                             C.POINTER(OptimizationStats)], C.c_int),
     ("symhip_optimizer_get_values", [_vp, F64P], C.c_int),
     ("symhip_optimizer_solver_info", [_vp, I64P], C.c_int),
+    ("symhip_optimizer_linear_solver_stats", [_vp, I64P], C.c_int),
+    ("symhip_optimizer_log_systems", [_vp, C.c_int32], C.c_int),
+    ("symhip_optimizer_logged_systems", [_vp], C.c_int32),
+    ("symhip_optimizer_rcs_pattern", [_vp, I32P, I32P], C.c_int32),
+    ("symhip_optimizer_logged_system", [_vp, C.c_int32, F64P, F64P], C.c_int),
+    ("symhip_optimizer_preconditioner_params", [_vp, F32P, F32P], C.c_int),
+    ("symhip_optimizer_preconditioner_loss_grad", [_vp, C.c_int32, F64P, C.c_int32, F32P, F32P],
+     C.c_int),
+    ("symhip_optimizer_train_preconditioner", [_vp, C.c_int32, C.c_float, C.c_int32, C.c_uint64,
+                                               C.c_int32, F32P], C.c_int),
+    ("symhip_optimizer_eval_preconditioner", [_vp, C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                              I32P, F64P], C.c_int),
+    ("symhip_batch_create", [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(OptimizerParams),
+                             C.c_int32, C.c_double, C.POINTER(_vp)], C.c_int),
+    ("symhip_batch_destroy", [_vp], C.c_int),
+    ("symhip_batch_storage_size", [_vp], C.c_int64),
+    ("symhip_batch_solve", [_vp, F64P, C.POINTER(OptimizationStats), C.POINTER(IterationStats),
+                            F64P, F64P], C.c_int),
     ("symhip_optimizer_iteration_times", [_vp, F64P, C.c_int32], C.c_int32),
     ("symhip_optimizer_enable_timing", [_vp, C.c_int32], C.c_int),
     ("symhip_optimizer_phase_times", [_vp, F64P, I64P], C.c_int),
@@ -412,8 +453,10 @@ class HipProblem:
 
 
 def solver_options(linear_solver=SOLVER_CHOLESKY, ordering=ORDERING_MIN_DEGREE,
-                   precision=PRECISION_F64, use_pcg=0, pcg_max_iters=100, pcg_rel_tol=1e-6):
-    return SolverOptions(linear_solver, ordering, precision, use_pcg, pcg_max_iters, pcg_rel_tol)
+                   precision=PRECISION_F64, use_pcg=0, pcg_max_iters=100, pcg_rel_tol=1e-6,
+                   pcg_preconditioner=0):
+    return SolverOptions(linear_solver, ordering, precision, use_pcg, pcg_max_iters, pcg_rel_tol,
+                         pcg_preconditioner)
 
 
 class HipOptimizer:
@@ -492,6 +535,127 @@ class HipOptimizer:
         return dict(n=int(out[0]), supernodes=int(out[1]), panel=int(out[2]),
                     levels=int(out[3]), flops=int(out[4]), blocks=int(out[5]),
                     pairs=int(out[6]), obs=int(out[7]))
+
+    # ---- learned PCG preconditioner (Schur + use_pcg) ----
+    def log_systems(self, max_systems):
+        check(hip_lib().symhip_optimizer_log_systems(self.h, max_systems))
+
+    def logged_systems(self):
+        return int(hip_lib().symhip_optimizer_logged_systems(self.h))
+
+    def rcs_pattern(self):
+        n = hip_lib().symhip_optimizer_rcs_pattern(self.h, None, None)
+        if n < 0:
+            check(-1)
+        row_ptr = np.zeros(n + 1, np.int32)
+        hip_lib().symhip_optimizer_rcs_pattern(self.h, ptr(row_ptr, C.c_int32), None)
+        col = np.zeros(int(row_ptr[-1]), np.int32)
+        hip_lib().symhip_optimizer_rcs_pattern(self.h, ptr(row_ptr, C.c_int32),
+                                               ptr(col, C.c_int32))
+        return row_ptr, col
+
+    def logged_system(self, idx):
+        row_ptr, col = self.rcs_pattern()
+        blocks = np.zeros(len(col) * 81)
+        rhs = np.zeros((len(row_ptr) - 1) * 9)
+        check(hip_lib().symhip_optimizer_logged_system(self.h, idx, ptr(blocks, C.c_double),
+                                                       ptr(rhs, C.c_double)))
+        return row_ptr, col, blocks, rhs
+
+    def preconditioner_params(self, set_params=None):
+        out = np.zeros(LEARNED_PRECOND_PARAMS, np.float32)
+        src = None if set_params is None else np.ascontiguousarray(set_params, np.float32)
+        check(hip_lib().symhip_optimizer_preconditioner_params(
+            self.h, ptr(out, C.c_float), None if src is None else ptr(src, C.c_float)))
+        return out
+
+    def preconditioner_loss_grad(self, idx, X):
+        X = np.asfortranarray(np.asarray(X, np.float64))  # probe p contiguous
+        P = X.shape[1]
+        loss = np.zeros(1, np.float32)
+        grad = np.zeros(LEARNED_PRECOND_PARAMS, np.float32)
+        check(hip_lib().symhip_optimizer_preconditioner_loss_grad(
+            self.h, idx, X.ctypes.data_as(F64P), P, ptr(loss, C.c_float), ptr(grad, C.c_float)))
+        return float(loss[0]), grad
+
+    def train_preconditioner(self, steps, lr=1e-3, probes=4, seed=0, n_systems=0):
+        losses = np.zeros(steps, np.float32)
+        check(hip_lib().symhip_optimizer_train_preconditioner(self.h, steps, lr, probes, seed,
+                                                              n_systems, ptr(losses, C.c_float)))
+        return losses
+
+    def eval_preconditioner(self, idx, rel_tol=1e-6, max_iters=1000, learned=False):
+        it = np.zeros(1, np.int32)
+        err = np.zeros(1)
+        check(hip_lib().symhip_optimizer_eval_preconditioner(
+            self.h, idx, rel_tol, max_iters, int(learned), ptr(it, C.c_int32),
+            ptr(err, C.c_double)))
+        return int(it[0]), float(err[0])
+
+    def linear_solver_stats(self):
+        out = np.zeros(4, np.int64)
+        check(hip_lib().symhip_optimizer_linear_solver_stats(self.h, ptr(out, C.c_int64)))
+        return dict(solves=int(out[0]), iterations=int(out[1]), fallbacks=int(out[2]),
+                    max_iterations=int(out[3]))
+
+
+class HipBatch:
+    """Many small localization problems solved by one persistent launch (one wavefront each)."""
+
+    def __init__(self, problems, params: OptimizerParams | None = None, precision=PRECISION_F64,
+                 epsilon=None):
+        self.problems = list(problems)
+        self.params = params or default_params()
+        eps = (self.problems[0].info.get("epsilon", 10 * np.finfo(np.float64).eps)
+               if epsilon is None else epsilon)
+        n = len(self.problems)
+        self._descs = (ProblemDesc * n)(*[p.desc() for p in self.problems])
+        h = C.c_void_p()
+        check(hip_lib().symhip_batch_create(self._descs, n, C.byref(self.params), precision, eps,
+                                            C.byref(h)))
+        self.h = h
+        self.sizes = [len(p.values) for p in self.problems]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            hip_lib().symhip_batch_destroy(self.h)
+            self.h = None
+
+    def solve(self, records=True):
+        n = len(self.problems)
+        vals = np.zeros(sum(self.sizes))
+        stats = (OptimizationStats * n)()
+        stride = self.params.iterations + 1
+        iters = (IterationStats * (n * stride))() if records else None
+        ms = np.zeros(n)
+        total = np.zeros(1)
+        check(hip_lib().symhip_batch_solve(self.h, ptr(vals, C.c_double), stats, iters,
+                                           ptr(ms, C.c_double), ptr(total, C.c_double)))
+        out, off = [], 0
+        for i in range(n):
+            st = stats[i]
+            rec = None
+            if records:
+                sub = (IterationStats * stride).from_address(C.addressof(iters) +
+                                                             i * stride * C.sizeof(IterationStats))
+                rec = iteration_records(sub, st.num_iterations)
+            out.append(dict(values=vals[off:off + self.sizes[i]].copy(), status=st.status,
+                            failure_reason=st.failure_reason, best_index=st.best_index,
+                            iterations=rec))
+            off += self.sizes[i]
+        return dict(problems=out, solve_ms=ms, total_ms=float(total[0]))
+
+
+def gen_localization_batch(n_problems=4096, seed=42, min_poses=5, max_poses=50,
+                           min_landmarks=5, max_landmarks=40, visible_fraction=0.3):
+    """Independent robot-3D-localization problems with uniform pose / landmark counts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_problems):
+        npose = int(rng.integers(min_poses, max_poses + 1))
+        nlm = int(rng.integers(min_landmarks, max_landmarks + 1))
+        out.append(gen_localization(npose, nlm, visible_fraction, seed=int(rng.integers(1 << 31))))
+    return out
 
 
 class HipCholesky:

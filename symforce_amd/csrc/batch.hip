@@ -1,0 +1,484 @@
+// Batched LM for small robot-3D-localization problems: one persistent launch, one wavefront per
+// problem running the whole Levenberg-Marquardt loop (the state-block bookkeeping of
+// LevenbergMarquardtSolver::Iterate, levenberg_marquardt_solver.tcc:138-343, with the shared
+// accept / lambda logic of lm_logic.h).
+//
+// Structure exploited: the optimized keys are the poses of one chain, odometry factors join
+// consecutive poses and matching factors touch one pose, so H is block tridiagonal with 6 x 6
+// blocks. Lane i linearizes pose i (its matching factors and the odometry factor to pose i + 1,
+// whose pose-(i + 1) terms are handed to lane i + 1 by a shuffle); lane 0 then solves the damped
+// system by block-tridiagonal Cholesky from LDS. Per state block and pose the linearization keeps
+// 63 doubles: the lower diagonal block (21), H(i + 1, i) (36, row-major) and the rhs (6).
+#include "./batch.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "./factor_traits.h"
+#include "./geo.h"
+#include "./lm_logic.h"
+#include "./problem.h"
+#include "./util.h"
+
+namespace symhip {
+
+namespace {
+
+constexpr int kLin = 63;
+constexpr int kMaxPoses = 64;
+
+struct BatchView {
+  int64_t total_poses;
+  const int32_t* pose_begin;  // [n_problems + 1] global pose index
+  const int32_t* pose_soff;   // [total_poses] offset of the initial pose value in cvals
+  const int32_t* match_ptr;   // [total_poses + 1]
+  const int32_t* match_args;  // 3 per match: landmark, measurement, sigma offsets in cvals
+  const int32_t* odo_args;    // 3 per pose: measurement, sigmas, epsilon offsets (-1: last pose)
+  const double* cvals;
+  double* vals;  // [3][total_poses][7]
+  double* lin;   // [3][total_poses][kLin]
+  double* maxd;  // [total_poses][6]
+  double* vout;  // storage copy; best poses written back
+  symhip_iteration_stats* recs;
+  int32_t rec_stride;
+  symhip_optimization_stats* stats;
+  int64_t* ticks;  // [n_problems][2]
+};
+
+__device__ __forceinline__ double wave_total(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }
+
+// Relinearize state block b: lane i < n writes pose i's D, H(i + 1, i), rhs. Returns the error
+// 0.5 |r|^2 of the problem (the same value in every lane).
+template <typename S>
+__device__ double linearize(const BatchView& v, int b, int64_t g, int lane, bool live) {
+  double D[21] = {}, gv[6] = {}, Bv[36] = {}, Dn[21] = {}, gn[6] = {};
+  double e = 0;
+  int has_next = 0;
+  if (live) {
+    const double* pose = v.vals + (b * v.total_poses + g) * 7;
+    for (int m = v.match_ptr[g]; m < v.match_ptr[g + 1]; ++m) {
+      const double* a[4] = {pose, v.cvals + v.match_args[3 * m], v.cvals + v.match_args[3 * m + 1],
+                            v.cvals + v.match_args[3 * m + 2]};
+      S r[3], J[3][6];
+      factor_matching<S>(a, r, J);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gv[i] += (double)(J[q][i] * r[q]);
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+          S h = 0;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) h += J[q][i] * J[q][j];
+          D[tri(i, j)] += (double)h;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) e += (double)r[q] * (double)r[q];
+    }
+    const int32_t* oa = v.odo_args + 3 * g;
+    if (oa[0] >= 0) {
+      has_next = 1;
+      const double* a[5] = {pose, pose + 7, v.cvals + oa[0], v.cvals + oa[1], v.cvals + oa[2]};
+      S r[6], J[6][12];
+      factor_odometry<S>(a, r, J);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        S ga = 0, gb = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          ga += J[q][i] * r[q];
+          gb += J[q][6 + i] * r[q];
+        }
+        gv[i] += (double)ga;
+        gn[i] = (double)gb;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          S hb = 0;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) hb += J[q][6 + i] * J[q][j];
+          Bv[i * 6 + j] = (double)hb;  // H(i + 1, i): rows of pose i + 1, cols of pose i
+        }
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+          S ha = 0, hn = 0;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            ha += J[q][i] * J[q][j];
+            hn += J[q][6 + i] * J[q][6 + j];
+          }
+          D[tri(i, j)] += (double)ha;
+          Dn[tri(i, j)] = (double)hn;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) e += (double)r[q] * (double)r[q];
+    }
+  }
+  // the odometry factor's pose-(i + 1) terms belong to the next lane
+  const int prev = __shfl_up(has_next, 1, 64);
+  const bool take = lane >= 1 && prev != 0;
+#pragma unroll
+  for (int k = 0; k < 21; ++k) {
+    const double t = __shfl_up(Dn[k], 1, 64);
+    if (take) D[k] += t;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double t = __shfl_up(gn[k], 1, 64);
+    if (take) gv[k] += t;
+  }
+  if (live) {
+    double* L = v.lin + (b * v.total_poses + g) * kLin;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) L[k] = D[k];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) L[21 + k] = Bv[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[57 + k] = gv[k];
+  }
+  return 0.5 * wave_total(e);
+}
+
+// Block-tridiagonal Cholesky H = L L^T (L block lower bidiagonal: C_i on the diagonal,
+// W_i = B_i C_i^-T below) and the two triangular solves. A: packed lower diagonal blocks
+// (overwritten by C_i), B: H(i + 1, i) row-major (overwritten by W_i), x: rhs in, H^-1 rhs out.
+__device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36], double (*x)[6]) {
+  for (int i = 0; i < n; ++i) {
+    double M[6][6];
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c <= r; ++c) M[r][c] = A[i][tri(r, c)];
+    if (i > 0) {
+      const double* W = B[i - 1];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c <= r; ++c) {
+          double s = 0;
+          for (int k = 0; k < 6; ++k) s += W[r * 6 + k] * W[c * 6 + k];
+          M[r][c] -= s;
+        }
+    }
+    for (int k = 0; k < 6; ++k) {
+      M[k][k] = sqrt(M[k][k]);
+      for (int r = k + 1; r < 6; ++r) M[r][k] /= M[k][k];
+      for (int r = k + 1; r < 6; ++r)
+        for (int c = k + 1; c <= r; ++c) M[r][c] -= M[r][k] * M[c][k];
+    }
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c <= r; ++c) A[i][tri(r, c)] = M[r][c];
+    if (i + 1 < n) {
+      double* W = B[i];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+          double w = W[r * 6 + c];
+          for (int k = 0; k < c; ++k) w -= W[r * 6 + k] * M[c][k];
+          W[r * 6 + c] = w / M[c][c];
+        }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double b[6];
+    for (int r = 0; r < 6; ++r) {
+      b[r] = x[i][r];
+      if (i > 0)
+        for (int k = 0; k < 6; ++k) b[r] -= B[i - 1][r * 6 + k] * x[i - 1][k];
+    }
+    for (int r = 0; r < 6; ++r) {
+      double s = b[r];
+      for (int k = 0; k < r; ++k) s -= A[i][tri(r, k)] * x[i][k];
+      x[i][r] = s / A[i][tri(r, r)];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double b[6];
+    for (int r = 0; r < 6; ++r) {
+      b[r] = x[i][r];
+      if (i + 1 < n)
+        for (int k = 0; k < 6; ++k) b[r] -= B[i][k * 6 + r] * x[i + 1][k];
+    }
+    for (int r = 5; r >= 0; --r) {
+      double s = b[r];
+      for (int k = r + 1; k < 6; ++k) s -= A[i][tri(k, r)] * x[i][k];
+      x[i][r] = s / A[i][tri(r, r)];
+    }
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimizer_params p,
+                                                      double eps) {
+  __shared__ double sA[kMaxPoses][21];
+  __shared__ double sB[kMaxPoses][36];
+  __shared__ double sx[kMaxPoses][6];
+  const int prob = blockIdx.x, lane = threadIdx.x;
+  const int64_t t_start = wall_clock64();
+  const int g0 = v.pose_begin[prob], n = v.pose_begin[prob + 1] - g0;
+  const bool live = lane < n;
+  const int64_t g = g0 + min(lane, n - 1);
+  const int64_t TP = v.total_poses;
+  // Reset: the start Values go into the New block (block 1)
+  if (live)
+    for (int k = 0; k < 7; ++k) v.vals[(TP + g) * 7 + k] = v.cvals[v.pose_soff[g] + k];
+  int init = 0, nw = 1, best = 0, fre = 2, iteration = -1, best_index = 0;
+  int status = 0, reason = 0, linearized = 0, nrec = 0;
+  bool have_max = false, done = false;
+  double err[3] = {0, 0, 0};
+  double lambda = p.initial_lambda, nu = p.dynamic_lambda_update_beta;
+  auto record = [&](const symhip_iteration_stats& r) {
+    if (lane == 0 && nrec < v.rec_stride) v.recs[(int64_t)prob * v.rec_stride + nrec] = r;
+    ++nrec;
+  };
+  auto relin = [&](int b) {
+    err[b] = linearize<S>(v, b, g, lane, live);
+    linearized |= 1 << b;
+  };
+  for (int it = 0; it < p.iterations && !done; ++it) {
+    const int t = init;
+    init = nw;
+    nw = t;
+    ++iteration;
+    if (!((linearized >> init) & 1)) {
+      relin(init);
+      if (best != init) {
+        if (best != nw) fre = best;
+        best = init;
+      }
+    }
+    if (iteration == 0) {
+      symhip_iteration_stats r{};
+      r.iteration = -1;
+      r.new_error = err[init];
+      r.current_lambda = lambda;
+      record(r);
+      if (!isfinite(err[init])) {
+        status = SYMHIP_STATUS_FAILED;
+        reason = SYMHIP_FAILURE_INITIAL_ERROR_NOT_FINITE;
+        break;
+      }
+    }
+    // DampHessian (levenberg_marquardt_solver.tcc:22-54) out of place, and the staged system
+    const double* Li = v.lin + (init * TP + g) * kLin;
+    double damp[6] = {}, u[6] = {};
+    if (live) {
+      for (int k = 0; k < 6; ++k) {
+        const double dk = Li[tri(k, k)];
+        double d = 0;
+        if (p.use_diagonal_damping) {
+          if (p.keep_max_diagonal_damping) {
+            const double md = have_max ? fmax(v.maxd[g * 6 + k], dk) : fmax(dk, p.diagonal_damping_min);
+            v.maxd[g * 6 + k] = md;
+            d = md * lambda;
+          } else {
+            d = fmax(dk, p.diagonal_damping_min) * lambda;
+          }
+        }
+        if (p.use_unit_damping) d += lambda;
+        damp[k] = d;
+      }
+      for (int k = 0; k < 21; ++k) sA[lane][k] = Li[k];
+      for (int k = 0; k < 6; ++k) sA[lane][tri(k, k)] += damp[k];
+      for (int k = 0; k < 36; ++k) sB[lane][k] = Li[21 + k];
+      for (int k = 0; k < 6; ++k) sx[lane][k] = Li[57 + k];
+    }
+    if (p.use_diagonal_damping && p.keep_max_diagonal_damping) have_max = true;
+    __syncthreads();
+    if (lane == 0) block_tridiagonal_solve(n, sA, sB, sx);
+    __syncthreads();
+    double lde = 0;
+    if (live) {
+      for (int k = 0; k < 6; ++k) {
+        u[k] = -sx[lane][k];
+        lde += u[k] * (Li[57 + k] - damp[k] * u[k]);
+      }
+      // UpdateNewFromInit: New = Init retracted by the update
+      const double* src = v.vals + (init * TP + g) * 7;
+      double* dst = v.vals + (nw * TP + g) * 7;
+      for (int k = 0; k < 7; ++k) dst[k] = src[k];
+      rot3_retract_inplace(dst, u, eps);
+      dst[4] += u[3];
+      dst[5] += u[4];
+      dst[6] += u[5];
+    }
+    __syncthreads();  // sA/sB/sx are rewritten next iteration
+    relin(nw);
+    lde = wave_total(lde);
+    const LmDecision d = lm_decide(err[init], err[nw], err[best], lde, lambda, nu, iteration, p,
+                                   0, 0.0, 0.0, 0.0, eps);
+    record(d.rec);
+    lambda = d.lambda;
+    nu = d.nu;
+    if (!d.accept) {
+      const int s = nw;
+      nw = init;
+      init = s;
+    } else {
+      if (d.set_best) {
+        if (best != nw) {
+          if (best != init) fre = best;
+          best = nw;
+        }
+        best_index = nrec - 1;
+      }
+      if (best == init) {
+        init = fre;
+        fre = best;
+      }
+    }
+    status = d.status;
+    reason = d.reason;
+    done = status != 0;
+  }
+  if (!done && status == 0) status = SYMHIP_STATUS_HIT_ITERATION_LIMIT;
+  if (live)
+    for (int k = 0; k < 7; ++k) v.vout[v.pose_soff[g] + k] = v.vals[(best * TP + g) * 7 + k];
+  if (lane == 0) {
+    v.stats[prob] = symhip_optimization_stats{nrec, best_index, status, reason};
+    v.ticks[2 * prob] = t_start;
+    v.ticks[2 * prob + 1] = wall_clock64();
+  }
+}
+
+}  // namespace
+
+BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n_problems,
+                                     const symhip_optimizer_params& params, bool compute_f32,
+                                     double epsilon)
+    : n_problems_(n_problems), p_(params), f32_(compute_f32), eps_(epsilon) {
+  SYMHIP_CHECK(descs && n_problems > 0, "batch: no problems");
+  SYMHIP_CHECK(params.iterations > 0, "batch: iterations must be positive");
+  SYMHIP_CHECK(!params.enable_bold_updates, "batch: bold updates are not supported");
+  std::vector<int32_t> pose_begin{0}, pose_soff, match_ptr{0}, match_args, odo_args;
+  std::vector<int64_t> storage_begin;
+  std::vector<double> cvals;
+  for (int32_t pi = 0; pi < n_problems; ++pi) {
+    HostProblem hp;
+    build_host_problem(&descs[pi], hp);
+    const int32_t n = (int32_t)hp.state_keys.size();
+    SYMHIP_CHECK(n >= 1 && n <= kMaxPoses, "batch: each problem needs 1..64 optimized poses");
+    std::vector<int32_t> pos_of(hp.n_keys, -1);
+    for (int32_t i = 0; i < n; ++i) {
+      SYMHIP_CHECK(hp.types[hp.state_keys[i]] == SYMHIP_TYPE_POSE3,
+                   "batch: optimized keys must be Pose3");
+      pos_of[hp.state_keys[i]] = i;
+    }
+    const int64_t base = (int64_t)cvals.size();
+    storage_begin.push_back(base);
+    cvals.insert(cvals.end(), hp.values.begin(), hp.values.end());
+    SYMHIP_CHECK(cvals.size() < (size_t)INT32_MAX, "batch: storage too large");
+    auto off = [&](int32_t key) { return (int32_t)(base + hp.storage_offset[key]); };
+    auto constant = [&](int32_t key) {
+      SYMHIP_CHECK(pos_of[key] < 0, "batch: factor constants must not be optimized");
+      return off(key);
+    };
+    std::vector<std::vector<int32_t>> matches(n);
+    std::vector<int32_t> odo(3 * (size_t)n, -1);
+    for (int32_t f = 0; f < hp.n_factors; ++f) {
+      const int32_t* a = hp.fargs.data() + hp.fargs_ptr[f];
+      if (hp.ftype[f] == SYMHIP_FACTOR_MATCHING) {
+        const int32_t i = pos_of[a[0]];
+        SYMHIP_CHECK(i >= 0, "batch: matching factor on a constant pose");
+        matches[i].insert(matches[i].end(), {constant(a[1]), constant(a[2]), constant(a[3])});
+      } else if (hp.ftype[f] == SYMHIP_FACTOR_ODOMETRY) {
+        const int32_t i = pos_of[a[0]], j = pos_of[a[1]];
+        SYMHIP_CHECK(i >= 0 && j == i + 1,
+                     "batch: odometry factors must join consecutive optimized poses (a, a + 1)");
+        SYMHIP_CHECK(odo[3 * i] < 0, "batch: two odometry factors leave the same pose");
+        odo[3 * i] = constant(a[2]);
+        odo[3 * i + 1] = constant(a[3]);
+        odo[3 * i + 2] = constant(a[4]);
+      } else {
+        throw Error("batch: only matching and odometry factors are supported");
+      }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+      pose_soff.push_back(off(hp.state_keys[i]));
+      match_args.insert(match_args.end(), matches[i].begin(), matches[i].end());
+      match_ptr.push_back((int32_t)(match_args.size() / 3));
+      odo_args.insert(odo_args.end(), odo.begin() + 3 * i, odo.begin() + 3 * i + 3);
+    }
+    pose_begin.push_back(pose_begin.back() + n);
+  }
+  total_poses_ = pose_begin.back();
+  cvals_host_size_ = cvals.size();
+  HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&ev0_));
+  HIP_OK(hipEventCreate(&ev1_));
+  pose_begin_.upload(pose_begin, stream_);
+  pose_soff_.upload(pose_soff, stream_);
+  match_ptr_.upload(match_ptr, stream_);
+  match_args_.upload(match_args.empty() ? std::vector<int32_t>{0} : match_args, stream_);
+  odo_args_.upload(odo_args, stream_);
+  storage_begin_.upload(storage_begin, stream_);
+  cvals_.upload(cvals, stream_);
+  vout_.upload(cvals, stream_);
+  vals_.resize((size_t)3 * total_poses_ * 7);
+  lin_.resize((size_t)3 * total_poses_ * kLin);
+  maxd_.resize((size_t)total_poses_ * 6);
+  recs_.resize((size_t)n_problems_ * (p_.iterations + 1));
+  stats_.resize(n_problems_);
+  ticks_.resize(2 * (size_t)n_problems_);
+  HIP_OK(hipStreamSynchronize(stream_));
+  build_graph();
+}
+
+BatchLocalization::~BatchLocalization() {
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void BatchLocalization::build_graph() {
+  BatchView v{total_poses_,      pose_begin_.get(), pose_soff_.get(), match_ptr_.get(),
+              match_args_.get(), odo_args_.get(),   cvals_.get(),     vals_.get(),
+              lin_.get(),        maxd_.get(),       vout_.get(),      recs_.get(),
+              p_.iterations + 1, stats_.get(),      ticks_.get()};
+  hipGraph_t graph;
+  HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  if (f32_)
+    hipLaunchKernelGGL(batch_lm_kernel<float>, dim3(n_problems_), dim3(64), 0, stream_, v, p_,
+                       eps_);
+  else
+    hipLaunchKernelGGL(batch_lm_kernel<double>, dim3(n_problems_), dim3(64), 0, stream_, v, p_,
+                       eps_);
+  HIP_OK(hipStreamEndCapture(stream_, &graph));
+  HIP_OK(hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(graph));
+}
+
+void BatchLocalization::solve(double* values_out, symhip_optimization_stats* stats_out,
+                              symhip_iteration_stats* iters_out, double* solve_ms,
+                              double* total_ms) {
+  HIP_OK(hipEventRecord(ev0_, stream_));
+  HIP_OK(hipGraphLaunch(graph_, stream_));
+  HIP_OK(hipEventRecord(ev1_, stream_));
+  if (values_out) vout_.download(values_out, cvals_host_size_, stream_);
+  if (stats_out) stats_.download(stats_out, n_problems_, stream_);
+  if (iters_out) recs_.download(iters_out, recs_.size(), stream_);
+  std::vector<int64_t> ticks;
+  if (solve_ms) {
+    ticks.resize(ticks_.size());
+    ticks_.download(ticks.data(), ticks.size(), stream_);
+  }
+  HIP_OK(hipStreamSynchronize(stream_));
+  if (total_ms) {
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    *total_ms = ms;
+  }
+  if (solve_ms) {
+    int dev = 0, khz = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    SYMHIP_CHECK(khz > 0, "batch: wall clock rate unavailable");
+    for (int32_t i = 0; i < n_problems_; ++i)
+      solve_ms[i] = (double)(ticks[2 * i + 1] - ticks[2 * i]) / (double)khz;
+  }
+}
+
+}  // namespace symhip

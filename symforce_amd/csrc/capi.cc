@@ -9,6 +9,7 @@
 #include "./common.h"
 #include "./chol_numeric.h"
 #include "./linearize.h"
+#include "./batch.h"
 #include "./optimizer.h"
 #include "./problem.h"
 
@@ -156,6 +157,7 @@ int symhip_optimizer_create(const symhip_problem_desc* desc, const symhip_optimi
       o.use_pcg = options->use_pcg;
       o.pcg_max_iters = options->pcg_max_iters;
       o.pcg_rel_tol = options->pcg_rel_tol;
+      o.pcg_preconditioner = options->pcg_preconditioner;
     }
     SYMHIP_CHECK(params->lambda_update_type == SYMHIP_LAMBDA_STATIC ||
                      params->lambda_update_type == SYMHIP_LAMBDA_DYNAMIC,
@@ -233,6 +235,57 @@ int symhip_optimizer_solver_info(symhip_optimizer_t o, int64_t* out) {
   return guarded([&] { o->opt->backend().info(out); });
 }
 
+int symhip_optimizer_linear_solver_stats(symhip_optimizer_t o, int64_t* out) {
+  return guarded([&] { o->opt->backend().linear_stats(out); });
+}
+
+int symhip_optimizer_log_systems(symhip_optimizer_t o, int32_t max) {
+  return guarded([&] { o->opt->backend().log_systems(max); });
+}
+
+int32_t symhip_optimizer_logged_systems(symhip_optimizer_t o) {
+  int32_t n = -1;
+  guarded([&] { n = o->opt->backend().logged_systems(); });
+  return n;
+}
+
+int32_t symhip_optimizer_rcs_pattern(symhip_optimizer_t o, int32_t* row_ptr, int32_t* col) {
+  int32_t n = -1;
+  guarded([&] { n = o->opt->backend().rcs_pattern(row_ptr, col); });
+  return n;
+}
+
+int symhip_optimizer_logged_system(symhip_optimizer_t o, int32_t idx, double* blocks,
+                                   double* rhs) {
+  return guarded([&] { o->opt->backend().logged_system(idx, blocks, rhs); });
+}
+
+int symhip_optimizer_preconditioner_params(symhip_optimizer_t o, float* get, const float* set) {
+  return guarded([&] { o->opt->backend().precond_params(get, set); });
+}
+
+int symhip_optimizer_preconditioner_loss_grad(symhip_optimizer_t o, int32_t idx, const double* X,
+                                              int32_t P, float* loss, float* grad) {
+  return guarded([&] {
+    SYMHIP_CHECK(X && loss, "null argument");
+    *loss = o->opt->backend().precond_loss_grad(idx, X, P, grad);
+  });
+}
+
+int symhip_optimizer_train_preconditioner(symhip_optimizer_t o, int32_t steps, float lr,
+                                          int32_t P, uint64_t seed, int32_t n_systems,
+                                          float* losses) {
+  return guarded([&] { o->opt->backend().train_precond(steps, lr, P, seed, n_systems, losses); });
+}
+
+int symhip_optimizer_eval_preconditioner(symhip_optimizer_t o, int32_t idx, double rel_tol,
+                                         int32_t max_iters, int32_t learned, int32_t* iterations,
+                                         double* step_err) {
+  return guarded([&] {
+    o->opt->backend().eval_precond(idx, rel_tol, max_iters, learned, iterations, step_err);
+  });
+}
+
 int symhip_cholesky_create(int32_t n, const int32_t* colptr, const int32_t* rowind,
                            const int32_t* node_ptr, int32_t n_nodes, int32_t ordering,
                            const int32_t* given_order, symhip_cholesky_t* out) {
@@ -249,8 +302,7 @@ int symhip_cholesky_create(int32_t n, const int32_t* colptr, const int32_t* rowi
       for (int32_t i = 0; i <= n; ++i) nodes[i] = i;
     }
     std::vector<int32_t> given;
-    symhip::Ordering ord = ordering == SYMHIP_ORDERING_NATURAL ? symhip::Ordering::kNatural
-                                                               : symhip::Ordering::kMinDegree;
+    symhip::Ordering ord = symhip::ordering_from_code(ordering);
     if (given_order) {
       given.assign(given_order, given_order + (int32_t)nodes.size() - 1);
       ord = symhip::Ordering::kGiven;
@@ -323,6 +375,49 @@ int symhip_cholesky_info(symhip_cholesky_t c, int64_t* out) {
     out[2] = c->sym.panel_size();
     out[3] = c->sym.n_levels;
     out[4] = (int64_t)c->sym.factor_flops;
+  });
+}
+
+}  // extern "C"
+
+#include "./gnn.h"
+static_assert(SYMHIP_LEARNED_PRECOND_PARAMS == symhip::LearnedPrecond::kParams,
+              "parameter count in the public header");
+
+struct symhip_batch {
+  std::unique_ptr<symhip::BatchLocalization> b;
+};
+
+extern "C" {
+
+int symhip_batch_create(const symhip_problem_desc* descs, int32_t n_problems,
+                        const symhip_optimizer_params* params, int32_t precision, double epsilon,
+                        symhip_batch_t* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(descs && params && out && n_problems > 0, "invalid arguments");
+    SYMHIP_CHECK(precision == SYMHIP_PRECISION_F64 || precision == SYMHIP_PRECISION_F32_JAC,
+                 "unknown precision");
+    auto h = std::make_unique<symhip_batch>();
+    h->b = std::make_unique<symhip::BatchLocalization>(
+        descs, n_problems, *params, precision == SYMHIP_PRECISION_F32_JAC, epsilon);
+    *out = h.release();
+  });
+}
+
+int symhip_batch_destroy(symhip_batch_t batch) {
+  return guarded([&] { delete batch; });
+}
+
+int64_t symhip_batch_storage_size(symhip_batch_t batch) {
+  return batch ? batch->b->storage_total() : -1;
+}
+
+int symhip_batch_solve(symhip_batch_t batch, double* values_out,
+                       symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
+                       double* solve_ms, double* total_ms) {
+  return guarded([&] {
+    SYMHIP_CHECK(batch, "invalid arguments");
+    batch->b->solve(values_out, stats_out, iters_out, solve_ms, total_ms);
   });
 }
 

@@ -34,43 +34,98 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
     panel[map[q]] = vals[q];
 }
 
-// Right-looking LDL^T of the 64 x 64 (padded) diagonal block held as 4 x 4 register patches:
-// thread (ty, tx) owns rows ty + 16x, cols tx + 16y. Step k = 16 kb + kk: the owners of column k
-// (tx == kk, register column kb) publish c_i = A(i, k) for i > k (zero otherwise) and
-// d = A(k, k) through LDS; every thread applies A(i, j) -= c_i c_j / d (a no-op outside the
-// trailing block because c vanishes there) and the owners replace column k by c / d. The
-// register column index kb is a compile-time constant (the kb loop is unrolled, the kk loop is
-// not), which keeps the patches in registers without unrolling all 64 steps.
-__device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*col)[kMaxCols],
-                                                 double* dval, int ty, int tx) {
+// Right-looking blocked LDL^T of the 64 x 64 (padded) diagonal block held as 4 x 4 register
+// patches: thread (ty, tx) owns rows ty + 16x, cols tx + 16y. Step m eliminates columns
+// k0 = 4m .. k0 + 3 (register column kb = k0 / 16, owner threads tx = k0 % 16 + u): the owners
+// publish the four columns through LDS (one barrier per step, double-buffered), every thread
+// factors the 4 x 4 diagonal block redundantly, forms w = c L4^-T for its rows and columns
+// (c = the published entries, L(i, k0 + u) = w_u / d_u) and applies
+// A(i, j) -= sum_u w_iu w_ju / d_u; the owners then store L (and D on the diagonal).
+__device__ __forceinline__ void ldlt4_block(const double (*cb)[kMaxCols], int k0, double (&d)[4],
+                                            double (&dinv)[4], double (&L4)[4][4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    double dd = cb[u][k0 + u];
+#pragma unroll
+    for (int v = 0; v < u; ++v) dd -= L4[u][v] * L4[u][v] * d[v];
+    d[u] = dd;
+    dinv[u] = 1.0 / dd;
+#pragma unroll
+    for (int r = u + 1; r < 4; ++r) {
+      double x = cb[u][k0 + r];
+#pragma unroll
+      for (int v = 0; v < u; ++v) x -= L4[r][v] * L4[u][v] * d[v];
+      L4[r][u] = x * dinv[u];
+    }
+  }
+}
+
+__device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*cb)[4][kMaxCols],
+                                                 int ty, int tx) {
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll 2
-    for (int kk = 0; kk < 16; ++kk) {
-      const int k = 16 * kb + kk, buf = kk & 1;
-      if (tx == kk) {
+#pragma unroll 1
+    for (int mm = 0; mm < 4; ++mm) {
+      const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
+      if (uo >= 0 && uo < 4) {
 #pragma unroll
-        for (int x = 0; x < 4; ++x) {
-          const int i = ty + 16 * x;
-          col[buf][i] = i > k ? a[x][kb] : 0.0;
-          if (i == k) dval[buf] = a[x][kb];
-        }
+        for (int x = 0; x < 4; ++x) cb[buf][uo][ty + 16 * x] = a[x][kb];
       }
       __syncthreads();
-      const double inv = 1.0 / dval[buf];
-      double ci[4], cj[4];
+      double d[4], dinv[4], L4[4][4];
+      ldlt4_block(cb[buf], k0, d, dinv, L4);
+      double wr[4][4], wc[4][4];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) ci[x] = col[buf][ty + 16 * x] * inv;
+      for (int x = 0; x < 4; ++x) {
+        const int i = ty + 16 * x;
+        const bool live = i >= k0 + 4;
 #pragma unroll
-      for (int y = 0; y < 4; ++y) cj[y] = col[buf][tx + 16 * y];
+        for (int u = 0; u < 4; ++u) {
+          double c = live ? cb[buf][u][i] : 0.0;
+#pragma unroll
+          for (int v = 0; v < u; ++v) c -= wr[x][v] * L4[u][v];
+          wr[x][u] = c;
+        }
+      }
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int j = tx + 16 * y;
+        const bool live = j >= k0 + 4;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          double c = live ? cb[buf][u][j] : 0.0;
+#pragma unroll
+          for (int v = 0; v < u; ++v) c -= wc[y][v] * L4[u][v];
+          wc[y][u] = c;
+        }
+      }
+      // wr <- L(i, k0 + u) = w_iu / d_u
 #pragma unroll
       for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) a[x][y] -= ci[x] * cj[y];
-      if (tx == kk) {
+        for (int u = 0; u < 4; ++u) wr[x][u] *= dinv[u];
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
-          if (ty + 16 * x > k) a[x][kb] = ci[x];
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) a[x][y] -= wr[x][u] * wc[y][u];
+      if (uo >= 0 && uo < 4) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int i = ty + 16 * x;
+          double v = a[x][kb];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (u != uo) continue;
+            if (i >= k0 + 4) v = wr[x][u];
+            else if (i == k0 + u) v = d[u];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (r > u && i == k0 + r) v = L4[r][u];
+          }
+          a[x][kb] = v;
+        }
       }
     }
   }
@@ -84,8 +139,7 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
                                                         double* __restrict__ panel,
                                                         double* __restrict__ l11,
                                                         double* __restrict__ D) {
-  __shared__ double col[2][kMaxCols];
-  __shared__ double dval[2];
+  __shared__ double cb[2][4][kMaxCols];
   __shared__ double L[kMaxCols][kMaxCols + 1];  // L[j][k] = L11(j, k) for j > k, else 0
   __shared__ double Dinv[kMaxCols];
   const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
@@ -95,17 +149,21 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
   double a[4][4], w[4][4];
   // diagonal block (padded with identity) and this chunk of the below rows, loaded together
+  // Unconditional loads from clamped (valid) addresses, then selects: a guarded load would be
+  // compiled as a branch and wait per element.
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
       const int i = ty + 16 * x, j = tx + 16 * y;
-      const bool in = i < n && j < n && i >= j;
-      a[x][y] = in ? P[(int64_t)j * ld + i] : (i == j ? 1.0 : 0.0);
+      const int ic = min(i, n - 1), jc = min(j, n - 1);
+      const double va = P[(int64_t)jc * ld + ic];
       const int r = chunk * 64 + i;
-      w[x][y] = (r < mb && j < n) ? P[(int64_t)j * ld + n + r] : 0.0;
+      const double vw = P[(int64_t)jc * ld + (mb > 0 ? n + min(r, mb - 1) : ic)];
+      a[x][y] = (i < n && j < n && i >= j) ? va : (i == j ? 1.0 : 0.0);
+      w[x][y] = (r < mb && j < n) ? vw : 0.0;
     }
-  ldlt64_registers(a, col, dval, ty, tx);
+  ldlt64_registers(a, cb, ty, tx);
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -125,26 +183,49 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
       }
   }
   __syncthreads();
-  // W L11^T = A21 column by column: column k of W is final at step k
+  // W L11^T = A21, four columns per step: the owners publish W(:, k0..k0+3) (all earlier
+  // blocks applied), every thread finishes them for its rows with the unit 4 x 4 block of L11 and
+  // applies them to its later columns.
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll 2
-    for (int kk = 0; kk < 16; ++kk) {
-      const int k = 16 * kb + kk, buf = kk & 1;
-      if (tx == kk) {
+#pragma unroll 1
+    for (int mm = 0; mm < 4; ++mm) {
+      const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
+      if (uo >= 0 && uo < 4) {
 #pragma unroll
-        for (int x = 0; x < 4; ++x) col[buf][ty + 16 * x] = w[x][kb];
+        for (int x = 0; x < 4; ++x) cb[buf][uo][ty + 16 * x] = w[x][kb];
       }
       __syncthreads();
-      double wk[4], lj[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x) wk[x] = col[buf][ty + 16 * x];
-#pragma unroll
-      for (int y = 0; y < 4; ++y) lj[y] = L[tx + 16 * y][k];
+      double wf[4][4], lj[4][4];
 #pragma unroll
       for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) w[x][y] -= wk[x] * lj[y];
+        for (int u = 0; u < 4; ++u) {
+          double c = cb[buf][u][ty + 16 * x];
+#pragma unroll
+          for (int v = 0; v < u; ++v) c -= wf[x][v] * L[k0 + u][k0 + v];
+          wf[x][u] = c;
+        }
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lj[y][u] = L[tx + 16 * y][k0 + u];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w[x][y] -= wf[x][u] * lj[y][u];
+      if (uo >= 0 && uo < 4) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          double v = wf[x][0];
+#pragma unroll
+          for (int u = 1; u < 4; ++u)
+            if (u == uo) v = wf[x][u];
+          w[x][kb] = v;
+        }
+      }
     }
   }
 #pragma unroll
@@ -176,21 +257,26 @@ __global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__
   const double* Ds = D + sd.col_begin;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   // stage both row tiles (all k at once: the loads are independent)
-  for (int idx = threadIdx.x; idx < kMaxCols * kTile; idx += blockDim.x) {
+  // (unconditional loads from clamped addresses; an update item exists only when mb > 0)
+#pragma unroll
+  for (int t = 0; t < kMaxCols * kTile / 256; ++t) {  // blockDim.x == 256
+    const int idx = threadIdx.x + 256 * t;
     const int k = idx / kTile, r = idx % kTile;
     const int ri = ti * kTile + r, rj = tj * kTile + r;
-    As[k][r] = (k < n && ri < mb) ? P[(int64_t)k * ld + n + ri] : 0.0;
-    Bs[k][r] = (k < n && rj < mb) ? P[(int64_t)k * ld + n + rj] * Ds[k] : 0.0;
+    const int kc = min(k, n - 1);
+    const double va = P[(int64_t)kc * ld + n + min(ri, mb - 1)];
+    const double vb = P[(int64_t)kc * ld + n + min(rj, mb - 1)] * Ds[kc];
+    As[k][r] = (k < n && ri < mb) ? va : 0.0;
+    Bs[k][r] = (k < n && rj < mb) ? vb : 0.0;
   }
   // scatter metadata for this thread's columns, fetched while the tiles load
   int64_t cbase[4];
   int32_t rbase[4];
 #pragma unroll
   for (int y = 0; y < 4; ++y) {
-    const int j = tj * kTile + tx + 16 * y;
-    const bool ok = j < mb;
-    cbase[y] = ok ? scat_col[sd.rows_off + n + j] : 0;
-    rbase[y] = ok ? scat_rel[sd.rows_off + n + j] : 0;
+    const int j = min(tj * kTile + tx + 16 * y, mb - 1);  // clamped; masked at the scatter
+    cbase[y] = scat_col[sd.rows_off + n + j];
+    rbase[y] = scat_rel[sd.rows_off + n + j];
   }
   __syncthreads();
   double acc[4][4] = {};
@@ -237,14 +323,20 @@ __global__ void dscale_kernel(int32_t n, const double* __restrict__ D, double* _
 // Stages the strictly lower part of L11 (n x n column-major at Lc, padded with zeros; unit
 // diagonal implied) into LDS: Ls[i][k].
 __device__ __forceinline__ void stage_l11(const double* Lc, int n, double (*Ls)[kMaxCols + 1]) {
-  for (int idx = threadIdx.x; idx < kMaxCols * kMaxCols; idx += blockDim.x) {
+#pragma unroll
+  for (int t = 0; t < kMaxCols * kMaxCols / 256; ++t) {  // blockDim.x == 256
+    const int idx = threadIdx.x + 256 * t;
     const int i = idx % kMaxCols, k = idx / kMaxCols;
-    Ls[i][k] = (i < n && k < n && i > k) ? Lc[(int64_t)k * n + i] : 0.0;
+    const double v = Lc[(int64_t)min(k, n - 1) * n + min(i, n - 1)];  // clamped, then masked
+    Ls[i][k] = (i < n && k < n && i > k) ? v : 0.0;
   }
 }
 
 // Forward: y_own = L11^-1 y_own (one wavefront, lane i holds y_i, L11 from LDS);
-// y_below -= L21 y_own with atomics (siblings at the same level share ancestors).
+// y_below -= L21 y_own with atomics (siblings at the same level share ancestors). A thread takes
+// one below-row and all 64 (padded) columns: its 64 panel loads are independent, and the first
+// row chunk's loads are issued before the triangular solve so they overlap it. Columns past n
+// are clamped to column n - 1 and multiplied by ys = 0.
 __global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ list,
                                                   const SnDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ rows,
@@ -256,8 +348,15 @@ __global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ li
   const int s = list[blockIdx.x];
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
-  const double* P = panel + sd.panel_off;
+  const double* P = panel + sd.panel_off + n;  // below rows of column 0
   double* yo = y + sd.col_begin;
+  const int32_t* rw = rows + sd.rows_off + n;
+  double lrow[kMaxCols];
+  {
+    const int i = min((int)threadIdx.x, max(mb - 1, 0));
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) lrow[k] = mb > 0 ? P[(int64_t)min(k, n - 1) * ld + i] : 0.0;
+  }
   stage_l11(l11 + sd.l11_off, n, Ls);
   double yi = 0;
   if (threadIdx.x < kMaxCols && threadIdx.x < n) yi = yo[threadIdx.x];
@@ -273,18 +372,22 @@ __global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ li
     ys[i] = i < n ? yi : 0.0;
   }
   __syncthreads();
-  const int32_t* rw = rows + sd.rows_off + n;
   for (int i = threadIdx.x; i < mb; i += blockDim.x) {
+    if (i >= (int)blockDim.x) {
+#pragma unroll
+      for (int k = 0; k < kMaxCols; ++k) lrow[k] = P[(int64_t)min(k, n - 1) * ld + i];
+    }
     double acc = 0;
-#pragma unroll 16
-    for (int k = 0; k < kMaxCols; ++k)
-      if (k < n) acc += P[(int64_t)k * ld + n + i] * ys[k];
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) acc += lrow[k] * ys[k];
     unsafeAtomicAdd(&y[rw[i]], -acc);
   }
 }
 
-// Backward: y_own -= L21^T y_below (waves split the columns, lanes the rows); then
-// y_own = L11^-T y_own by one wavefront from LDS.
+// Backward: y_own -= L21^T y_below, then y_own = L11^-T y_own by one wavefront from LDS. Wave w
+// takes columns w, w + 4, ..., w + 60 at once (16 accumulators per lane, lanes over the rows), so
+// each row step issues 16 independent coalesced column loads; the 16 sums are reduced across the
+// wave at the end. Columns past n are clamped and their sums dropped.
 __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ list,
                                                   const SnDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ rows,
@@ -297,21 +400,34 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ li
   const int s = list[blockIdx.x];
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
-  const double* P = panel + sd.panel_off;
+  const double* P = panel + sd.panel_off + n;
   double* yo = y + sd.col_begin;
   const int32_t* rw = rows + sd.rows_off + n;
   stage_l11(l11 + sd.l11_off, n, Ls);
   const bool cached = mb <= 1024;
-  if (cached)
+  if (cached) {
+#pragma unroll 4
     for (int i = threadIdx.x; i < mb; i += blockDim.x) yb[i] = y[rw[i]];
+  }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int k = wave; k < n; k += nw) {
-    double acc = 0;
-    for (int i = lane; i < mb; i += 64)
-      acc += P[(int64_t)k * ld + n + i] * (cached ? yb[i] : y[rw[i]]);
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if (lane == 0) ys[k] = yo[k] - acc;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int kPer = kMaxCols / 4;
+  double acc[kPer] = {};
+  int64_t coff[kPer];
+#pragma unroll
+  for (int c = 0; c < kPer; ++c) coff[c] = (int64_t)min(wave + 4 * c, n - 1) * ld;
+  for (int i = lane; i < mb; i += 64) {
+    const double yv = cached ? yb[i] : y[rw[i]];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) acc[c] += P[coff[c] + i] * yv;
+  }
+#pragma unroll
+  for (int c = 0; c < kPer; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int k = wave + 4 * c;
+    if (lane == 0 && k < n) ys[k] = yo[k] - v;
   }
   __syncthreads();
   if (threadIdx.x < 64) {

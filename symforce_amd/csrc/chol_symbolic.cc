@@ -65,6 +65,159 @@ std::vector<int32_t> min_degree_order(const std::vector<std::vector<int32_t>>& a
   return order;
 }
 
+namespace {
+
+// Recursive state for nested_dissection_order.
+struct Dissector {
+  const std::vector<std::vector<int32_t>>& adj;
+  const std::vector<int32_t>& weight;
+  int32_t leaf;
+  std::vector<int32_t> order, stamp, dist;
+  int32_t epoch = 0;
+
+  Dissector(const std::vector<std::vector<int32_t>>& a, const std::vector<int32_t>& w, int32_t l)
+      : adj(a), weight(w), leaf(l), stamp(a.size(), -1), dist(a.size(), -1) {
+    order.reserve(a.size());
+  }
+
+  // Marks verts as the current subgraph (stamp == epoch).
+  void mark(const std::vector<int32_t>& verts) {
+    ++epoch;
+    for (int32_t v : verts) stamp[v] = epoch;
+  }
+
+  // BFS inside the marked subgraph; returns nodes in BFS order, dist filled for them.
+  std::vector<int32_t> bfs(int32_t root) {
+    std::vector<int32_t> q{root};
+    dist[root] = 0;
+    const int32_t seen = -epoch - 2;  // distinct per epoch, never a valid stamp
+    int32_t* st = stamp.data();
+    st[root] = seen;
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int32_t v = q[h];
+      for (int32_t u : adj[v])
+        if (st[u] == epoch) {
+          st[u] = seen;
+          dist[u] = dist[v] + 1;
+          q.push_back(u);
+        }
+    }
+    for (int32_t v : q) st[v] = epoch;  // restore membership
+    return q;
+  }
+
+  void leaf_order(const std::vector<int32_t>& verts) {
+    // minimum degree on the induced subgraph, in local indices
+    std::vector<int32_t> local(verts.size());
+    std::vector<std::vector<int32_t>> sub(verts.size());
+    std::vector<int32_t> w(verts.size());
+    mark(verts);
+    std::vector<int32_t>& pos = dist;  // reuse as local index map
+    for (size_t i = 0; i < verts.size(); ++i) pos[verts[i]] = (int32_t)i;
+    for (size_t i = 0; i < verts.size(); ++i) {
+      w[i] = weight[verts[i]];
+      for (int32_t u : adj[verts[i]])
+        if (stamp[u] == epoch) sub[i].push_back(pos[u]);
+    }
+    for (int32_t i : min_degree_order(sub, w)) order.push_back(verts[i]);
+  }
+
+  void dissect(std::vector<int32_t> verts) {
+    if ((int32_t)verts.size() <= leaf) {
+      leaf_order(verts);
+      return;
+    }
+    mark(verts);
+    // connected components: dissect each on its own
+    {
+      std::vector<std::vector<int32_t>> comps;
+      const int32_t my_epoch = epoch;
+      for (int32_t v : verts) {
+        if (stamp[v] != my_epoch) continue;  // already in a found component
+        std::vector<int32_t> comp = bfs(v);
+        for (int32_t u : comp) stamp[u] = -1;  // remove from the pool
+        comps.push_back(std::move(comp));
+      }
+      if (comps.size() > 1) {
+        for (auto& c : comps) dissect(std::move(c));
+        return;
+      }
+      mark(verts);
+    }
+    // pseudo-peripheral root: BFS again from the farthest node while the eccentricity grows
+    std::vector<int32_t> q = bfs(verts[0]);
+    int32_t ecc = dist[q.back()];
+    for (int it = 0; it < 4; ++it) {
+      std::vector<int32_t> q2 = bfs(q.back());
+      const int32_t e2 = dist[q2.back()];
+      q.swap(q2);  // dist now belongs to q
+      if (e2 <= ecc) break;
+      ecc = e2;
+    }
+    // level weights
+    const int32_t depth = dist[q.back()];
+    if (depth < 2) {  // (near-)clique: nothing to separate
+      leaf_order(verts);
+      return;
+    }
+    std::vector<int64_t> lw(depth + 1, 0);
+    int64_t total = 0;
+    for (int32_t v : q) {
+      lw[dist[v]] += weight[v];
+      total += weight[v];
+    }
+    // separator level: among levels 1..depth-1, the smallest level whose removal keeps both sides
+    // within [25%, 75%] of the weight; the median level if none does
+    int64_t below = 0;
+    int32_t best = -1, median = 1;
+    int64_t best_w = INT64_MAX;
+    for (int32_t l = 0; l <= depth; ++l) {
+      if (l >= 1 && l <= depth - 1) {
+        const int64_t above = total - below - lw[l];
+        if (below <= total / 2) median = l;
+        if (4 * below >= total && 4 * above >= total && lw[l] < best_w) {
+          best_w = lw[l];
+          best = l;
+        }
+      }
+      below += lw[l];
+    }
+    if (best < 0) best = median;
+    std::vector<int32_t> A, B, sep;
+    for (int32_t v : q) {
+      if (dist[v] < best) A.push_back(v);
+      else if (dist[v] > best) B.push_back(v);
+      else sep.push_back(v);
+    }
+    dissect(std::move(A));
+    dissect(std::move(B));
+    leaf_order(sep);
+  }
+};
+
+}  // namespace
+
+Ordering ordering_from_code(int32_t code) {
+  switch (code) {
+    case 0: return Ordering::kNatural;
+    case 1: return Ordering::kMinDegree;
+    case 2: return Ordering::kNestedDissection;
+  }
+  throw Error("symhip: unknown ordering " + std::to_string(code));
+}
+
+std::vector<int32_t> nested_dissection_order(const std::vector<std::vector<int32_t>>& adj,
+                                             const std::vector<int32_t>& weight,
+                                             int32_t leaf_nodes) {
+  const int32_t n = (int32_t)adj.size();
+  Dissector d(adj, weight, std::max(1, leaf_nodes));
+  std::vector<int32_t> all(n);
+  std::iota(all.begin(), all.end(), 0);
+  d.dissect(std::move(all));
+  SYMHIP_CHECK((int32_t)d.order.size() == n, "internal: nested dissection lost nodes");
+  return d.order;
+}
+
 void scalar_etree_counts(int32_t n, const int32_t* colptr, const int32_t* rowind,
                          const std::vector<int32_t>& perm, std::vector<int32_t>& parent,
                          std::vector<int32_t>& counts) {
@@ -132,6 +285,8 @@ void chol_symbolic(int32_t n, const int32_t* colptr, const int32_t* rowind,
     SYMHIP_CHECK(given_node_order && (int32_t)given_node_order->size() == nn,
                  "given node ordering has the wrong size");
     s.node_order = *given_node_order;
+  } else if (ordering == Ordering::kNestedDissection) {
+    s.node_order = nested_dissection_order(adj, dim, 24);
   } else {
     s.node_order = min_degree_order(adj, dim);
   }

@@ -11,7 +11,7 @@
 
 namespace symhip {
 
-enum class Ordering { kNatural = 0, kMinDegree = 1, kGiven = 2 };
+enum class Ordering { kNatural = 0, kMinDegree = 1, kGiven = 2, kNestedDissection = 3 };
 
 struct CholSymbolic {
   int32_t n = 0;        // scalar dimension
@@ -51,6 +51,18 @@ struct CholSymbolic {
 // node weights are the scalar dims. Returns the elimination order (new -> old node).
 std::vector<int32_t> min_degree_order(const std::vector<std::vector<int32_t>>& adj,
                                       const std::vector<int32_t>& weight);
+
+// Nested-dissection ordering of a node graph (the family METIS belongs to; the reference orders
+// with METIS_NodeND through Eigen::MetisOrdering): recursive vertex separators taken from a BFS
+// level structure rooted at a pseudo-peripheral node, parts first, separator last; subgraphs of
+// at most leaf_nodes nodes are ordered by minimum degree. The shallower elimination tree gives the
+// level-scheduled numeric factorization more independent supernodes per level.
+// Maps a symhip_ordering code to the enum (throws on an unknown code).
+Ordering ordering_from_code(int32_t code);
+
+std::vector<int32_t> nested_dissection_order(const std::vector<std::vector<int32_t>>& adj,
+                                             const std::vector<int32_t>& weight,
+                                             int32_t leaf_nodes);
 
 // Builds the symbolic factorization. `colptr/rowind` is the lower pattern (full diagonal
 // present). node_ptr partitions 0..n into consecutive node ranges. When ordering == kGiven,

@@ -21,6 +21,7 @@ struct SolverOptions {
   int use_pcg = 0;   // Schur path: preconditioned CG on the reduced camera system
   int pcg_max_iters = 100;
   double pcg_rel_tol = 1e-6;
+  int pcg_preconditioner = 0;  // 0 block Jacobi, 1 learned correction
 };
 
 // A linear system backend owns the linearization storage of the three LM state blocks and the
@@ -48,9 +49,39 @@ class SystemBackend {
   // out[0]=factorized dim, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=flops,
   // out[5]=RCS blocks, out[6]=observation pairs, out[7]=observations
   virtual void info(int64_t* out8) const = 0;
+  // Iterative linear solver statistics: out[0]=solves, out[1]=total iterations, out[2]=fallbacks
+  // to the direct factorization, out[3]=most iterations in one solve (zeros for direct solvers).
+  virtual void linear_stats(int64_t* out4) const { out4[0] = out4[1] = out4[2] = out4[3] = 0; }
+  // Learned PCG preconditioner (Schur path with PCG only; the defaults reject the call).
+  // Logging keeps copies of the next max reduced camera systems (full block storage) and their
+  // right-hand sides as they are assembled.
+  virtual void log_systems(int max) { unsupported(); }
+  virtual int logged_systems() const { return 0; }
+  // Block pattern of the reduced camera system: returns the node count; fills row_ptr (n + 1)
+  // and col (row_ptr[n]) when non-null.
+  virtual int rcs_pattern(int32_t* row_ptr, int32_t* col) const { return unsupported(), 0; }
+  virtual void logged_system(int idx, double* blocks, double* rhs) const { unsupported(); }
+  virtual void precond_params(float* get, const float* set) { unsupported(); }
+  // Loss and gradient of the network on logged system idx for host probes X (P columns).
+  virtual float precond_loss_grad(int idx, const double* X, int P, float* grad) {
+    return unsupported(), 0.0f;
+  }
+  virtual void train_precond(int steps, float lr, int P, uint64_t seed, int n_systems,
+                             float* losses) {
+    unsupported();
+  }
+  // PCG on logged system idx (its own rhs) with block Jacobi (learned == 0) or the network:
+  // iterations to rel_tol and the relative error of the step against the supernodal LDL^T solve.
+  virtual void eval_precond(int idx, double rel_tol, int max_iters, int learned, int* iters,
+                            double* step_err) {
+    unsupported();
+  }
   void set_timer(PhaseTimer* t) { timer_ = t; }
 
  protected:
+  [[noreturn]] static void unsupported() {
+    throw Error("symhip: the learned preconditioner needs the Schur solver with use_pcg");
+  }
   PhaseTimer* timer_ = nullptr;
 };
 

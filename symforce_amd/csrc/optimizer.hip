@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "./geo.h"
+#include "./lm_logic.h"
 #include "./optimizer.h"
 #include "./util.h"
 
@@ -50,61 +51,16 @@ __global__ void lm_first_record_kernel(LmDevState* st, int init) {
 // The accept / reject / lambda logic of Iterate (levenberg_marquardt_solver.tcc:239-340).
 __global__ void lm_decide_kernel(LmDevState* st, int init, int nw, int best, int iteration,
                                  symhip_optimizer_params p, int have_last_update, double eps) {
-  const double e0 = st->err[init], e1 = st->err[nw];
-  const double rel = (e0 - e1) / (e0 + eps);
-  const double nel = e0 + 0.5 * st->lde;
-  const double gain = (e0 - e1) / (e0 - nel);
-  symhip_iteration_stats rec{};
-  rec.iteration = iteration;
-  rec.current_lambda = st->lambda;
-  rec.new_error = e1;
-  rec.new_error_linear = nel;
-  rec.relative_reduction = rel;
-  int status = 0, reason = 0;
-  if (rel > -p.early_exit_min_reduction / 10 && rel < p.early_exit_min_reduction)
-    status = SYMHIP_STATUS_SUCCESS;
-  else if (e1 < p.early_exit_min_absolute_error)
-    status = SYMHIP_STATUS_SUCCESS;
-  bool accept = rel > 0;
-  double angle = 0;
-  if (p.enable_bold_updates && have_last_update && !accept) {
-    angle = (st->nl2 > 0 && st->nu2 > 0) ? st->dotlu / (sqrt(st->nl2) * sqrt(st->nu2)) : 0.0;
-    accept = ((1 - angle) * (1 - angle) * e1) <= st->err[best];
-  }
-  if (!accept && st->lambda >= p.lambda_upper_bound) {
-    status = SYMHIP_STATUS_FAILED;
-    reason = SYMHIP_FAILURE_LAMBDA_OUT_OF_BOUNDS;
-  }
-  double lambda = st->lambda, nu = st->nu;
-  int set_best = 0;
-  if (!accept) {
-    if (p.lambda_update_type == SYMHIP_LAMBDA_STATIC) {
-      lambda *= p.lambda_up_factor;
-    } else {
-      lambda *= nu;
-      nu *= 2;
-    }
-  } else {
-    if (p.lambda_update_type == SYMHIP_LAMBDA_STATIC) {
-      lambda *= p.lambda_down_factor;
-    } else {
-      lambda *= fmax(1.0 / p.dynamic_lambda_update_gamma,
-                     1.0 - (p.dynamic_lambda_update_beta - 1) *
-                               pow(2 * gain - 1, (double)p.dynamic_lambda_update_p));
-      nu = 2;
-    }
-    set_best = e1 <= st->err[best];
-  }
-  lambda = fmin(fmax(lambda, p.lambda_lower_bound), p.lambda_upper_bound);
-  rec.update_angle_change = angle;
-  rec.update_accepted = accept ? 1 : 0;
-  st->rec = rec;
-  st->lambda = lambda;
-  st->nu = nu;
-  st->status = status;
-  st->reason = reason;
-  st->accept = accept ? 1 : 0;
-  st->set_best = set_best;
+  const LmDecision d = lm_decide(st->err[init], st->err[nw], st->err[best], st->lde, st->lambda,
+                                 st->nu, iteration, p, have_last_update, st->nl2, st->nu2,
+                                 st->dotlu, eps);
+  st->rec = d.rec;
+  st->lambda = d.lambda;
+  st->nu = d.nu;
+  st->status = d.status;
+  st->reason = d.reason;
+  st->accept = d.accept;
+  st->set_best = d.set_best;
 }
 
 // Values::Retract over the optimized keys (opt/values.cc:314-327).
@@ -223,7 +179,7 @@ class CscBackend final : public SystemBackend {
     for (int32_t k : p.state_keys) node_ptr.push_back(node_ptr.back() + tangent_dim(p.types[k]));
     CholSymbolic sym;
     chol_symbolic(p.N, p.colptr.data(), p.rowind.data(), node_ptr,
-                  o.ordering == 0 ? Ordering::kNatural : Ordering::kMinDegree, nullptr, 64, sym);
+                  ordering_from_code(o.ordering), nullptr, 64, sym);
     chol_.setup(sym, p.colptr.data(), p.rowind.data(), s);
     N_ = p.N;
     M_ = p.M;
@@ -296,6 +252,9 @@ std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const Solv
 DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_optimizer_params& p,
                                  const SolverOptions& o, double epsilon)
     : p_(p), o_(o), epsilon_(epsilon) {
+  SYMHIP_CHECK(o.precision == 0, "symhip: only SYMHIP_PRECISION_F64 is implemented");
+  SYMHIP_CHECK(!o.use_pcg || o.linear_solver == kSolverSchur,
+               "symhip: PCG is available on the Schur path only");
   build_host_problem(d, host_);
   HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   backend_ = o.linear_solver == kSolverSchur ? make_schur_backend(host_, o, stream_)

@@ -4,9 +4,12 @@
 // same engines the reference tests use, so problems built here from a reference test's recipe
 // (test/symforce_optimizer_test.cc:79-131, :183-264, :276-348) reproduce its inputs exactly.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <limits>
 #include <random>
 #include <string>
@@ -574,6 +577,152 @@ symhip_generated* symhip_gen_localization(int32_t n_poses, int32_t n_landmarks,
   g->info["epsilon"] = kDefaultEpsilon;
   g->info["c_dim"] = 0;
   return g;
+}
+
+// BAL text format (bundle_adjustment_in_the_large.cc:60-118): header "cameras points
+// observations", observations "camera point x y", cameras "rx ry rz tx ty tz f k1 k2" (angle-axis
+// rotation of cam_T_world, converted with Rot3::FromTangent at the default epsilon), points
+// "x y z". Keys as the reference example's gen/keys.h: 'c' pose, 'i' intrinsics, 'p' point,
+// 'P' pixel (super = index), 'e' epsilon. huber_k > 0 selects the Huber-whitened factor with
+// threshold key 'h', else the plain SnavelyReprojectionFactor. Returns NULL on a read error.
+symhip_generated* symhip_load_bal(const char* path, double huber_k) {
+  std::ifstream f(path);
+  if (!f) return nullptr;
+  int64_t nc = 0, np = 0, no = 0;
+  if (!(f >> nc >> np >> no) || nc <= 0 || np <= 0 || no <= 0) return nullptr;
+  struct O {
+    int64_t cam, pt;
+    double u, v;
+  };
+  std::vector<O> obs((size_t)no);
+  for (auto& o : obs)
+    if (!(f >> o.cam >> o.pt >> o.u >> o.v) || o.cam < 0 || o.cam >= nc || o.pt < 0 || o.pt >= np)
+      return nullptr;
+  auto* g = new symhip_generated();
+  std::vector<int32_t> kc(nc), ki(nc), kp(np);
+  std::vector<std::array<double, 3>> intr(nc);
+  for (int64_t i = 0; i < nc; ++i) {
+    double r[3], t[3];
+    if (!(f >> r[0] >> r[1] >> r[2] >> t[0] >> t[1] >> t[2] >> intr[i][0] >> intr[i][1] >>
+          intr[i][2])) {
+      delete g;
+      return nullptr;
+    }
+    Pose p;
+    p.q = qexp(r, kDefaultEpsilon);
+    std::copy(t, t + 3, p.t);
+    kc[i] = g->add_pose('c', kInvalid, i, p);
+  }
+  for (int64_t i = 0; i < nc; ++i)
+    ki[i] = g->add_key('i', kInvalid, i, SYMHIP_TYPE_VECTOR3, intr[i].data(), 3);
+  for (int64_t j = 0; j < np; ++j) {
+    double v[3];
+    if (!(f >> v[0] >> v[1] >> v[2])) {
+      delete g;
+      return nullptr;
+    }
+    kp[j] = g->add_key('p', kInvalid, j, SYMHIP_TYPE_VECTOR3, v, 3);
+  }
+  const bool robust = huber_k > 0;
+  const int32_t kh = robust ? g->add_scalar('h', kInvalid, kInvalid, huber_k) : -1;
+  const int32_t ke = g->add_scalar('e', kInvalid, kInvalid, kDefaultEpsilon);
+  for (int64_t o = 0; o < no; ++o) {
+    const int32_t kx = g->add_vec('P', kInvalid, o, SYMHIP_TYPE_VECTOR2, {obs[o].u, obs[o].v});
+    if (robust)
+      g->add_factor(SYMHIP_FACTOR_SNAVELY_HUBER, {kc[obs[o].cam], ki[obs[o].cam], kp[obs[o].pt], kx,
+                                                  kh, ke});
+    else
+      g->add_factor(SYMHIP_FACTOR_SNAVELY, {kc[obs[o].cam], ki[obs[o].cam], kp[obs[o].pt], kx, ke});
+  }
+  g->info["epsilon"] = kDefaultEpsilon;
+  g->info["c_dim"] = 3.0 * (double)np;
+  g->info["n_cams"] = (double)nc;
+  g->info["n_points"] = (double)np;
+  g->info["n_obs"] = (double)no;
+  return g;
+}
+
+// Writes the Snavely observations, cameras and points of a problem in BAL text (rotation as the
+// angle-axis logarithm of the cam_T_world quaternion, cameras and points numbered in key
+// order). Factors other than the Snavely reprojection (e.g. gauge priors) have no BAL form
+// and are skipped. Returns 0, or -1 when there is no Snavely factor or the file cannot be written.
+int symhip_save_bal(const symhip_problem_desc* d, const char* path) {
+  if (!d || !path) return -1;
+  std::vector<int64_t> off((size_t)d->n_keys + 1, 0);
+  for (int32_t k = 0; k < d->n_keys; ++k) {
+    int w = 0;
+    switch (d->key_types[k]) {
+      case SYMHIP_TYPE_SCALAR: w = 1; break;
+      case SYMHIP_TYPE_ROT3: w = 4; break;
+      case SYMHIP_TYPE_POSE3: w = 7; break;
+      case SYMHIP_TYPE_VECTOR2: w = 2; break;
+      case SYMHIP_TYPE_VECTOR3: w = 3; break;
+      case SYMHIP_TYPE_VECTOR4: w = 4; break;
+      case SYMHIP_TYPE_VECTOR6: w = 6; break;
+      case SYMHIP_TYPE_MATRIX33: w = 9; break;
+      case SYMHIP_TYPE_MATRIX66: w = 36; break;
+      default: return -1;
+    }
+    off[k + 1] = off[k] + w;
+  }
+  // cameras and points numbered in key order (the order they were added to the Values)
+  std::vector<int32_t> cam_of_pose((size_t)d->n_keys, -1), pt_key;
+  std::vector<int64_t> num((size_t)d->n_keys, -1);
+  std::vector<char> is_cam((size_t)d->n_keys, 0), is_pt((size_t)d->n_keys, 0);
+  for (int32_t fi = 0; fi < d->n_factors; ++fi) {
+    const int t = d->factor_types[fi];
+    if (t != SYMHIP_FACTOR_SNAVELY && t != SYMHIP_FACTOR_SNAVELY_HUBER) continue;
+    const int32_t* a = d->factor_args + d->factor_arg_ptr[fi];
+    is_cam[a[0]] = 1;
+    cam_of_pose[a[0]] = a[1];
+    is_pt[a[2]] = 1;
+  }
+  std::vector<int32_t> cams, pts;
+  for (int32_t k = 0; k < d->n_keys; ++k) {
+    if (is_cam[k]) {
+      num[k] = (int64_t)cams.size() / 2;
+      cams.push_back(k);
+      cams.push_back(cam_of_pose[k]);
+    } else if (is_pt[k]) {
+      num[k] = (int64_t)pts.size();
+      pts.push_back(k);
+    }
+  }
+  struct O {
+    int64_t cam, pt;
+    int32_t px;
+  };
+  std::vector<O> obs;
+  for (int32_t fi = 0; fi < d->n_factors; ++fi) {
+    const int t = d->factor_types[fi];
+    if (t != SYMHIP_FACTOR_SNAVELY && t != SYMHIP_FACTOR_SNAVELY_HUBER) continue;
+    const int32_t* a = d->factor_args + d->factor_arg_ptr[fi];
+    obs.push_back({num[a[0]], num[a[2]], a[3]});
+  }
+  if (obs.empty()) return -1;
+  std::ofstream f(path);
+  if (!f) return -1;
+  f << std::setprecision(17);
+  f << cams.size() / 2 << " " << pts.size() << " " << obs.size() << "\n";
+  for (const O& o : obs) {
+    const double* x = d->values + off[o.px];
+    f << o.cam << " " << o.pt << " " << x[0] << " " << x[1] << "\n";
+  }
+  for (size_t c = 0; c < cams.size(); c += 2) {
+    const double* p = d->values + off[cams[c]];
+    Q q{p[0], p[1], p[2], p[3]};
+    if (q.w < 0) q = {-q.x, -q.y, -q.z, -q.w};
+    const double vn = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+    const double sc = vn > 0 ? 2.0 * std::atan2(vn, q.w) / vn : 2.0;
+    const double* k = d->values + off[cams[c + 1]];
+    f << sc * q.x << " " << sc * q.y << " " << sc * q.z << " " << p[4] << " " << p[5] << " "
+      << p[6] << " " << k[0] << " " << k[1] << " " << k[2] << "\n";
+  }
+  for (int32_t k : pts) {
+    const double* v = d->values + off[k];
+    f << v[0] << " " << v[1] << " " << v[2] << "\n";
+  }
+  return f.good() ? 0 : -1;
 }
 
 void symhip_gen_free(symhip_generated* g) { delete g; }

@@ -15,6 +15,8 @@
 #include <numeric>
 
 #include "./chol_numeric.h"
+#include "./gnn.h"
+#include "./pcg.h"
 #include "./factor_traits.h"
 #include "./optimizer.h"
 #include "./util.h"
@@ -30,11 +32,14 @@ struct ObsView {
   const int32_t* node;      // [n_obs]
 };
 
-// Per point (thread): evaluate its observations, write residuals, W_o (ND x 3 row-major) and the
-// point block V_p (lower 6: 00 10 11 20 21 22) and g_p (3).
+// Per observation (thread): evaluate the factor, write its residual and W_o = J_node^T J_point
+// (ND x 3 row-major), and add its J_point^T J_point (lower 6: 00 10 11 20 21 22) and
+// J_point^T r to the point's V_p and g_p. Observations are point-major, so a point's terms sit in
+// consecutive lanes: a segmented shuffle scan sums them inside the wavefront and the last lane of
+// each segment adds the sum with one atomic per value (V and g start at zero).
 template <int FTC, int ND>
-__global__ void __launch_bounds__(128) lin_points_kernel(int32_t n_points,
-                                                         const int32_t* __restrict__ pt_obs_ptr,
+__global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
+                                                         const int32_t* __restrict__ obs_point,
                                                          ObsView ov, const double* __restrict__ values,
                                                          double* __restrict__ residual,
                                                          double* __restrict__ W,
@@ -42,10 +47,13 @@ __global__ void __launch_bounds__(128) lin_points_kernel(int32_t n_points,
                                                          double* __restrict__ gp) {
   using F = FT<FTC>;
   static_assert(F::T == ND + 3, "landmark factor layout");
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_points) return;
-  double v[6] = {}, g[3] = {};
-  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = o < n_obs;
+  double acc[9] = {};  // v[6], g[3]
+  int p = -1 - lane;   // distinct per dead lane so dead lanes never join a segment
+  if (live) {
+    p = obs_point[o];
     const double* a[F::NA];
 #pragma unroll
     for (int k = 0; k < F::NA; ++k) a[k] = values + ov.arg_off[k * ov.n_obs + o];
@@ -59,31 +67,45 @@ __global__ void __launch_bounds__(128) lin_points_kernel(int32_t n_points,
     for (int i = 0; i < ND; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        double acc = 0;
+        double h = 0;
 #pragma unroll
-        for (int q = 0; q < F::D; ++q) acc += J[q][i] * J[q][ND + k];
-        Wo[i * 3 + k] = acc;
+        for (int q = 0; q < F::D; ++q) h += J[q][i] * J[q][ND + k];
+        Wo[i * 3 + k] = h;
       }
     int idx = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      double acc = 0;
+      double gi = 0;
 #pragma unroll
-      for (int q = 0; q < F::D; ++q) acc += J[q][ND + i] * r[q];
-      g[i] += acc;
+      for (int q = 0; q < F::D; ++q) gi += J[q][ND + i] * r[q];
+      acc[6 + i] = gi;
 #pragma unroll
       for (int j = 0; j <= i; ++j) {
         double h = 0;
 #pragma unroll
         for (int q = 0; q < F::D; ++q) h += J[q][ND + i] * J[q][ND + j];
-        v[idx++] += h;
+        acc[idx++] = h;
       }
     }
   }
+  // inclusive segmented scan over lanes with equal p (segments are contiguous)
 #pragma unroll
-  for (int i = 0; i < 6; ++i) V[(int64_t)p * 6 + i] = v[i];
+  for (int off = 1; off < 64; off <<= 1) {
+    const int pu = __shfl_up(p, off, 64);
+    const bool take = lane >= off && pu == p;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) gp[(int64_t)p * 3 + i] = g[i];
+    for (int k = 0; k < 9; ++k) {
+      const double u = __shfl_up(acc[k], off, 64);
+      if (take) acc[k] += u;
+    }
+  }
+  const int pn = __shfl_down(p, 1, 64);
+  if (live && (lane == 63 || pn != p)) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&V[(int64_t)p * 6 + k], acc[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&gp[(int64_t)p * 3 + k], acc[6 + k]);
+  }
 }
 
 // Per camera node (wavefront): recompute its observations' node Jacobians and reduce the node
@@ -229,17 +251,13 @@ __global__ void damp_schur_kernel(int32_t n_nodes, int32_t n_points,
   if (i < nb) dnode[i] = d; else dpoint[i - nb] = d;
 }
 
-// Per point: damped V_p -> L_p (Cholesky), h_p = L_p^-1 g_p, Z_o = W_o L_p^-T.
-template <int ND>
-__global__ void __launch_bounds__(128) point_elim_kernel(int32_t n_points,
-                                                         const int32_t* __restrict__ pt_obs_ptr,
-                                                         const double* __restrict__ V,
-                                                         const double* __restrict__ dpoint,
-                                                         const double* __restrict__ gp,
-                                                         const double* __restrict__ W,
-                                                         double* __restrict__ Lp,
-                                                         double* __restrict__ hp,
-                                                         double* __restrict__ Z) {
+// Per point: damped V_p -> L_p (Cholesky), h_p = L_p^-1 g_p.
+__global__ void __launch_bounds__(256) point_factor_kernel(int32_t n_points,
+                                                           const double* __restrict__ V,
+                                                           const double* __restrict__ dpoint,
+                                                           const double* __restrict__ gp,
+                                                           double* __restrict__ Lp,
+                                                           double* __restrict__ hp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_points) return;
   const double* v = V + (int64_t)p * 6;
@@ -250,7 +268,6 @@ __global__ void __launch_bounds__(128) point_elim_kernel(int32_t n_points,
   const double l11 = sqrt(a11 - l10 * l10);
   const double l21 = (a21 - l20 * l10) / l11;
   const double l22 = sqrt(a22 - l20 * l20 - l21 * l21);
-  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
   double* L = Lp + (int64_t)p * 6;
   L[0] = l00;
   L[1] = l10;
@@ -259,30 +276,45 @@ __global__ void __launch_bounds__(128) point_elim_kernel(int32_t n_points,
   L[4] = l21;
   L[5] = l22;
   const double* g = gp + (int64_t)p * 3;
-  const double h0 = g[0] * i00;
-  const double h1 = (g[1] - l10 * h0) * i11;
-  const double h2 = (g[2] - l20 * h0 - l21 * h1) * i22;
+  const double h0 = g[0] / l00;
+  const double h1 = (g[1] - l10 * h0) / l11;
+  const double h2 = (g[2] - l20 * h0 - l21 * h1) / l22;
   hp[p * 3 + 0] = h0;
   hp[p * 3 + 1] = h1;
   hp[p * 3 + 2] = h2;
-  // Z = W L^-T: each row w of W solves z L^T = w  (forward substitution with L)
-  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
-    const double* Wo = W + (int64_t)o * ND * 3;
-    double* Zo = Z + (int64_t)o * ND * 3;
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      const double z0 = Wo[i * 3 + 0] * i00;
-      const double z1 = (Wo[i * 3 + 1] - l10 * z0) * i11;
-      const double z2 = (Wo[i * 3 + 2] - l20 * z0 - l21 * z1) * i22;
-      Zo[i * 3 + 0] = z0;
-      Zo[i * 3 + 1] = z1;
-      Zo[i * 3 + 2] = z2;
-    }
-  }
 }
 
-// One wavefront per S block (a, b): S = [Bdiag + damping if a == b] + Bfac - sum_pairs Z_a Z_b^T,
-// written into the LDL^T panels at (base + col * ld + row) or transposed.
+// Per (observation, camera row): Z_o(i, :) = W_o(i, :) L_p^-T (forward substitution with L_p).
+// Consecutive threads walk consecutive rows of W and Z, so the 216 B per observation move as
+// contiguous runs.
+template <int ND>
+__global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
+                                                         const int32_t* __restrict__ obs_point,
+                                                         const double* __restrict__ Lp,
+                                                         const double* __restrict__ W,
+                                                         double* __restrict__ Z) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_obs * ND) return;
+  const int64_t o = t / ND;
+  const double* L = Lp + (int64_t)obs_point[o] * 6;
+  const double* w = W + t * 3;
+  const double z0 = w[0] / L[0];
+  const double z1 = (w[1] - L[1] * z0) / L[2];
+  const double z2 = (w[2] - L[3] * z0 - L[4] * z1) / L[5];
+  double* z = Z + t * 3;
+  z[0] = z0;
+  z[1] = z1;
+  z[2] = z2;
+}
+
+// One wavefront per S block (a, b): S = [Bdiag + damping if a == b] + Bfac - sum_pairs Z_x Z_y^T.
+// The pair sum runs on the FP64 matrix cores, one v_mfma_f64_16x16x4f64 per observation pair
+// (x, y): lane l feeds A(i = l % 16, k = l / 16) = Z_x(i, k) and B(k, j = l % 16) = Z_y(j, k) for
+// k < 3 (k = 3 and rows i >= ND are zero padding) and holds D(l / 16 + 4 r, l % 16) in
+// accumulator register r (the FP64 16x16x4 output interleaves rows across the four lane groups). The pair indices are wave-uniform (scalar loads); 8 pairs' Z loads are
+// issued before their MFMAs. Output: into the LDL^T panels at (base + col * ld + row) or
+// transposed, or (sfull != null) into full symmetric block storage, block row-major, at
+// f_lo[blk] and its mirror f_up[blk] (transposed).
 template <int ND>
 __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     int32_t n_blocks, const int32_t* __restrict__ blk_a, const int32_t* __restrict__ blk_b,
@@ -290,45 +322,57 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     const double* __restrict__ Z, const double* __restrict__ Bdiag,
     const double* __restrict__ Bfac, const double* __restrict__ dnode,
     const int64_t* __restrict__ panel_base, const int32_t* __restrict__ panel_ld,
-    const int32_t* __restrict__ panel_trans, double* __restrict__ panel) {
+    const int32_t* __restrict__ panel_trans, double* __restrict__ panel,
+    double* __restrict__ sfull, const int32_t* __restrict__ f_lo,
+    const int32_t* __restrict__ f_up) {
+  static_assert(ND <= 16, "camera node wider than the MFMA tile");
+  typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int NE = ND * ND;
-  const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int blk = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (blk >= n_blocks) return;
   const int a = blk_a[blk], b = blk_b[blk];
-  // each lane owns entries e = lane and lane + 64 (ND*ND <= 128)
-  double acc[2] = {0, 0};
-  int ei[2], ej[2];
+  const int row = lane & 15, kg = lane >> 4;
+  const bool live = row < ND && kg < 3;
+  const int zoff = live ? row * 3 + kg : 0;
+  d4 acc[4] = {};
+  const int q_begin = pair_ptr[blk], q_end = pair_ptr[blk + 1];
+  for (int q = q_begin; q < q_end; q += 8) {
+    double av[8], bv[8];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int e = lane + 64 * t;
-    ei[t] = e < NE ? e / ND : 0;
-    ej[t] = e < NE ? e % ND : 0;
-  }
-  for (int q = pair_ptr[blk]; q < pair_ptr[blk + 1]; ++q) {
-    const double* Za = Z + (int64_t)pairs[2 * q] * ND * 3;
-    const double* Zb = Z + (int64_t)pairs[2 * q + 1] * ND * 3;
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const double* x = Za + ei[t] * 3;
-      const double* y = Zb + ej[t] * 3;
-      acc[t] += x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+    for (int h = 0; h < 8; ++h) {
+      const bool okq = q + h < q_end;
+      const int qq = min(q + h, q_end - 1);  // in range; the tail's products are masked
+      const int xo = pairs[2 * qq], yo = pairs[2 * qq + 1];
+      const double za = Z[(int64_t)xo * (ND * 3) + zoff];
+      const double zb = Z[(int64_t)yo * (ND * 3) + zoff];
+      av[h] = (okq && live) ? za : 0.0;
+      bv[h] = (okq && live) ? zb : 0.0;
     }
-  }
-  const int64_t base = panel_base[blk];
-  const int ld = panel_ld[blk];
-  const bool trans = panel_trans[blk] != 0;
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int e = lane + 64 * t;
-    if (e >= NE) continue;
-    const int i = ei[t], j = ej[t];
+    for (int h = 0; h < 8; ++h)
+      acc[h & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc[h & 3], 0, 0, 0);
+  }
+  const d4 sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  const int j = row;
+  if (j >= ND) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = kg + 4 * r;
+    if (i >= ND) continue;
     if (a == b && i < j) continue;
-    double s = Bfac[(int64_t)blk * NE + i * ND + j] - acc[t];
+    double s = Bfac[(int64_t)blk * NE + i * ND + j] - sum[r];
     if (a == b) s += Bdiag[(int64_t)a * NE + i * ND + j] + (i == j ? dnode[a * ND + i] : 0.0);
-    // entry (row a*ND+i, col b*ND+j) of S
-    const int64_t pos = trans ? base + (int64_t)i * ld + j : base + (int64_t)j * ld + i;
-    panel[pos] = s;
+    if (sfull) {
+      sfull[(int64_t)f_lo[blk] * NE + i * ND + j] = s;
+      sfull[(int64_t)f_up[blk] * NE + j * ND + i] = s;
+    } else {
+      // entry (row a*ND+i, col b*ND+j) of S
+      const int64_t base = panel_base[blk];
+      const int ld = panel_ld[blk];
+      const int64_t pos = panel_trans[blk] ? base + (int64_t)i * ld + j : base + (int64_t)j * ld + i;
+      panel[pos] = s;
+    }
   }
 }
 
@@ -433,6 +477,22 @@ class SchurBackend final : public SystemBackend {
     out[6] = n_pairs_;
     out[7] = n_obs_;
   }
+  void log_systems(int max) override;
+  int logged_systems() const override { return (int)logged_S_.size(); }
+  int rcs_pattern(int32_t* row_ptr, int32_t* col) const override;
+  void logged_system(int idx, double* blocks, double* rhs) const override;
+  void precond_params(float* get, const float* set) override;
+  float precond_loss_grad(int idx, const double* X, int P, float* grad) override;
+  void train_precond(int steps, float lr, int P, uint64_t seed, int n_systems,
+                     float* losses) override;
+  void eval_precond(int idx, double rel_tol, int max_iters, int learned, int* iters,
+                    double* step_err) override;
+  void linear_stats(int64_t* out) const override {
+    out[0] = pcg_solves_;
+    out[1] = pcg_iters_;
+    out[2] = pcg_fallbacks_;
+    out[3] = pcg_max_seen_;
+  }
 
  private:
   template <int ND>
@@ -462,6 +522,23 @@ class SchurBackend final : public SystemBackend {
   DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, v_, y_, partials_;
   SupernodalLDLT chol_;
   std::unique_ptr<CscLinearizer> csc_;  // reference-layout H for best-linearization output
+  // Iterative form: PCG on the reduced camera system in full block storage, falling back to the
+  // supernodal LDL^T when it does not reach the tolerance.
+  bool use_pcg_ = false, pcg_strict_ = false;
+  int pcg_max_iters_ = 100;
+  double pcg_rel_tol_ = 1e-6;
+  BlockPcg pcg_;
+  DBuf<int32_t> f_lo_, f_up_;
+  std::vector<int32_t> full_row_ptr_, full_col_;
+  // learned correction of the block-Jacobi preconditioner and the logged training systems
+  bool use_learned_ = false;
+  LearnedPrecond gnn_;
+  int log_max_ = 0;
+  std::vector<DBuf<double>> logged_S_, logged_v_;
+  DBuf<double> probe_x_, probe_y_, exact_y_;
+  template <int ND>
+  void panels_from_full(const double* S, hipStream_t s);
+  int64_t pcg_solves_ = 0, pcg_iters_ = 0, pcg_fallbacks_ = 0, pcg_max_seen_ = 0;
 };
 
 namespace {
@@ -715,7 +792,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   for (int32_t a = 0; a <= n_nodes_; ++a) node_ptr[a] = a * nd_;
   CholSymbolic sym;
   chol_symbolic(nB, scp.data(), sri.data(), node_ptr,
-                o.ordering == 0 ? Ordering::kNatural : Ordering::kMinDegree, nullptr, 64, sym);
+                ordering_from_code(o.ordering), nullptr, 64, sym);
   // panel position of each S block's (0,0) entry
   std::vector<int32_t> sn_of_col(nB);
   for (int32_t k = 0; k < sym.n_snodes; ++k)
@@ -744,6 +821,48 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     ptrans[k] = trans ? 1 : 0;
   }
   chol_.setup(sym, scp.data(), sri.data(), s);
+  if (o.use_pcg) {
+    SYMHIP_CHECK(o.pcg_max_iters >= 1 && o.pcg_rel_tol > 0 && o.pcg_rel_tol < 1,
+                 "PCG needs pcg_max_iters >= 1 and 0 < pcg_rel_tol < 1");
+    SYMHIP_CHECK(o.use_pcg == 1 || o.use_pcg == 2, "use_pcg must be 0, 1 (truncated) or 2 (strict)");
+    use_pcg_ = true;
+    pcg_strict_ = o.use_pcg == 2;
+    pcg_max_iters_ = o.pcg_max_iters;
+    pcg_rel_tol_ = o.pcg_rel_tol;
+    // full symmetric block pattern: row a holds b for every lower block (a, b) and (b, a)
+    std::vector<std::vector<int32_t>> full(n_nodes_);
+    for (int32_t k = 0; k < n_blocks_; ++k) {
+      full[blk_a[k]].push_back(blk_b[k]);
+      if (blk_a[k] != blk_b[k]) full[blk_b[k]].push_back(blk_a[k]);
+    }
+    std::vector<int32_t> frp(n_nodes_ + 1, 0), fcol, fdiag(n_nodes_);
+    for (int32_t a = 0; a < n_nodes_; ++a) {
+      std::sort(full[a].begin(), full[a].end());
+      frp[a + 1] = frp[a] + (int32_t)full[a].size();
+      fcol.insert(fcol.end(), full[a].begin(), full[a].end());
+    }
+    auto full_index = [&](int32_t a, int32_t b) {
+      const auto it = std::lower_bound(full[a].begin(), full[a].end(), b);
+      return frp[a] + (int32_t)(it - full[a].begin());
+    };
+    std::vector<int32_t> flo(n_blocks_), fup(n_blocks_);
+    for (int32_t k = 0; k < n_blocks_; ++k) {
+      flo[k] = full_index(blk_a[k], blk_b[k]);
+      fup[k] = full_index(blk_b[k], blk_a[k]);
+    }
+    for (int32_t a = 0; a < n_nodes_; ++a) fdiag[a] = full_index(a, a);
+    pcg_.setup(nd_, n_nodes_, frp, fcol, fdiag, s);
+    full_row_ptr_ = frp;
+    full_col_ = fcol;
+    SYMHIP_CHECK(o.pcg_preconditioner == 0 || o.pcg_preconditioner == 1,
+                 "unknown pcg_preconditioner");
+    if (nd_ == LearnedPrecond::kNd) gnn_.setup(n_nodes_, frp, fcol, fdiag, s);
+    use_learned_ = o.pcg_preconditioner == 1;
+    SYMHIP_CHECK(!use_learned_ || gnn_.ready(),
+                 "the learned preconditioner needs 9-dof camera nodes");
+    f_lo_.upload(flo, s);
+    f_up_.upload(fup, s);
+  }
   // Camera-only factors
   std::vector<std::vector<int32_t>> bf_by_type(kNumFactorTypes);
   for (int32_t f : b_factors) bf_by_type[P.ftype[f]].push_back(f);
@@ -827,8 +946,10 @@ void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
     if constexpr (FT<C>::T == ND + 3 && FT<C>::NL >= 2) {
       {
         PhaseScope ps(timer_, SYMHIP_PHASE_LIN_POINTS, s);
-        hipLaunchKernelGGL((lin_points_kernel<C, ND>), dim3(grid_for(n_points_, 128)), dim3(128),
-                           0, s, n_points_, pt_obs_ptr_.get(), ov, values, residual_[b].get(),
+        fill_zero(V_[b].get(), (int64_t)n_points_ * 6, s);
+        fill_zero(gp_[b].get(), (int64_t)n_points_ * 3, s);
+        hipLaunchKernelGGL((lin_points_kernel<C, ND>), dim3(grid_for(n_obs_, 256)), dim3(256), 0,
+                           s, n_obs_, obs_point_.get(), ov, values, residual_[b].get(),
                            W_[b].get(), V_[b].get(), gp_[b].get());
       }
       PhaseScope ps(timer_, SYMHIP_PHASE_LIN_CAMS, s);
@@ -876,31 +997,71 @@ template <int ND>
 void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_POINT_ELIM, s);
-    hipLaunchKernelGGL((point_elim_kernel<ND>), dim3(grid_for(n_points_, 128)), dim3(128), 0, s,
-                       n_points_, pt_obs_ptr_.get(), V_[b].get(), dpoint_.get(), gp_[b].get(),
-                       W_[b].get(), Lp_.get(), hp_.get(), Z_.get());
+    hipLaunchKernelGGL(point_factor_kernel, dim3(grid_for(n_points_, 256)), dim3(256), 0, s,
+                       n_points_, V_[b].get(), dpoint_.get(), gp_[b].get(), Lp_.get(), hp_.get());
+    hipLaunchKernelGGL((point_elim_kernel<ND>), dim3(grid_for((int64_t)n_obs_ * ND, 256)),
+                       dim3(256), 0, s, n_obs_, obs_point_.get(), Lp_.get(), W_[b].get(), Z_.get());
   }
-  fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
-  {
+  auto assemble = [&](bool full) {
     PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);
+    if (!full) fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
     hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(grid_for((int64_t)n_blocks_ * 64, 256)),
                        dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
                        pairs_.get(), Z_.get(), Bdiag_[b].get(), Bfac_[b].get(), dnode_.get(),
-                       panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels());
-  }
+                       panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels(),
+                       full ? pcg_.blocks() : nullptr, f_lo_.get(), f_up_.get());
+  };
+  auto direct = [&]() {
+    {
+      PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
+      chol_.factorize_assembled(s);
+    }
+    PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
+    chol_.solve(v_.get(), y_.get(), s);
+  };
+  assemble(use_pcg_);
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
     hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
                        dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
                        obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get());
   }
-  {
-    PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
-    chol_.factorize_assembled(s);
+  if (use_pcg_ && (int)logged_S_.size() < log_max_) {
+    DBuf<double> Sc, vc;
+    Sc.resize((size_t)pcg_.n_blocks() * ND * ND);
+    vc.resize((size_t)n_nodes_ * ND);
+    HIP_OK(hipMemcpyAsync(Sc.get(), pcg_.blocks(), Sc.size() * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(vc.get(), v_.get(), vc.size() * sizeof(double), hipMemcpyDeviceToDevice,
+                          s));
+    logged_S_.push_back(std::move(Sc));
+    logged_v_.push_back(std::move(vc));
   }
-  {
-    PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
-    chol_.solve(v_.get(), y_.get(), s);
+  if (use_pcg_) {
+    PcgResult res;
+    {
+      PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
+      if (use_learned_)
+        gnn_.preconditioner(pcg_.blocks(), pcg_.precond(), s);
+      else
+        pcg_.build_preconditioner(nullptr, s);
+      res = pcg_.solve(v_.get(), y_.get(), pcg_max_iters_, pcg_rel_tol_, s);
+    }
+    ++pcg_solves_;
+    pcg_iters_ += res.iterations;
+    pcg_max_seen_ = std::max<int64_t>(pcg_max_seen_, res.iterations);
+    // Truncated mode (use_pcg == 1) keeps a CG iterate that did not reach the tolerance: every
+    // CG iterate lowers the quadratic model, and the LM gain ratio judges the step. The direct
+    // LDL^T takes over when CG stalls (breakdown, no residual reduction) or, in strict mode
+    // (use_pcg == 2), whenever the tolerance was not reached.
+    const bool stalled = res.breakdown || !(res.rel_residual < 1.0);
+    if (stalled || (pcg_strict_ && !res.converged)) {
+      ++pcg_fallbacks_;
+      assemble(false);
+      direct();
+    }
+  } else {
+    direct();
   }
   PhaseScope ps(timer_, SYMHIP_PHASE_BACK_SUB, s);
   hipLaunchKernelGGL(scatter_neg_kernel, dim3(grid_for(n_nodes_ * ND, 256)), dim3(256), 0, s,
@@ -920,6 +1081,144 @@ void SchurBackend::hessian_csc(int b, double* out, hipStream_t s) {
   (void)out;
   (void)s;
   throw Error("symhip: reference-layout Hessian output is not available on the Schur path");
+}
+
+namespace {
+// Full symmetric block storage -> LDL^T panels (the lower blocks, lower half of diagonal ones).
+template <int ND>
+__global__ void full_to_panels_kernel(int32_t n_blocks, const int32_t* __restrict__ blk_a,
+                                      const int32_t* __restrict__ blk_b,
+                                      const int32_t* __restrict__ f_lo,
+                                      const int64_t* __restrict__ panel_base,
+                                      const int32_t* __restrict__ panel_ld,
+                                      const int32_t* __restrict__ panel_trans,
+                                      const double* __restrict__ S, double* __restrict__ panel) {
+  constexpr int NE = ND * ND;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_blocks * NE) return;
+  const int blk = (int)(t / NE), e = (int)(t % NE), i = e / ND, j = e % ND;
+  if (blk_a[blk] == blk_b[blk] && i < j) return;
+  const int64_t base = panel_base[blk];
+  const int ld = panel_ld[blk];
+  const int64_t pos = panel_trans[blk] ? base + (int64_t)i * ld + j : base + (int64_t)j * ld + i;
+  panel[pos] = S[(int64_t)f_lo[blk] * NE + e];
+}
+}  // namespace
+
+template <int ND>
+void SchurBackend::panels_from_full(const double* S, hipStream_t s) {
+  fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
+  hipLaunchKernelGGL((full_to_panels_kernel<ND>), dim3(grid_for((int64_t)n_blocks_ * ND * ND, 256)),
+                     dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), f_lo_.get(),
+                     panel_base_.get(), panel_ld_.get(), panel_trans_.get(), S, chol_.panels());
+  HIP_OK(hipGetLastError());
+}
+
+void SchurBackend::log_systems(int max) {
+  SYMHIP_CHECK(use_pcg_, "logging reduced camera systems needs use_pcg");
+  SYMHIP_CHECK(max >= 0, "max must be non-negative");
+  log_max_ = max;
+  logged_S_.clear();
+  logged_v_.clear();
+}
+
+int SchurBackend::rcs_pattern(int32_t* row_ptr, int32_t* col) const {
+  SYMHIP_CHECK(use_pcg_, "the block pattern is built with use_pcg");
+  if (row_ptr) std::copy(full_row_ptr_.begin(), full_row_ptr_.end(), row_ptr);
+  if (col) std::copy(full_col_.begin(), full_col_.end(), col);
+  return n_nodes_;
+}
+
+void SchurBackend::logged_system(int idx, double* blocks, double* rhs) const {
+  SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
+  hipStream_t s = nullptr;
+  if (blocks) logged_S_[idx].download(blocks, logged_S_[idx].size(), s);
+  if (rhs) logged_v_[idx].download(rhs, logged_v_[idx].size(), s);
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+void SchurBackend::precond_params(float* get, const float* set) {
+  SYMHIP_CHECK(gnn_.ready(), "the learned preconditioner needs the Schur solver with use_pcg "
+                             "and 9-dof camera nodes");
+  if (set) gnn_.set_params(set, nullptr);
+  if (get) gnn_.get_params(get, nullptr);
+}
+
+float SchurBackend::precond_loss_grad(int idx, const double* X, int P, float* grad) {
+  SYMHIP_CHECK(gnn_.ready(), "learned preconditioner unavailable");
+  SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
+  SYMHIP_CHECK(P >= 1 && P <= 7, "1..7 probe vectors");
+  hipStream_t s = nullptr;
+  const int64_t n9 = (int64_t)n_nodes_ * nd_;
+  probe_x_.upload(X, (size_t)n9 * P, s);
+  probe_y_.resize((size_t)n9 * P);
+  for (int p = 0; p < P; ++p)
+    pcg_.multiply_with(logged_S_[idx].get(), probe_x_.get() + p * n9, probe_y_.get() + p * n9, s);
+  const float loss = gnn_.loss_grad(logged_S_[idx].get(), probe_x_.get(), probe_y_.get(), P, s);
+  if (grad) {
+    HIP_OK(hipMemcpyAsync(grad, gnn_.grad(), sizeof(float) * LearnedPrecond::kParams,
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+  }
+  return loss;
+}
+
+void SchurBackend::train_precond(int steps, float lr, int P, uint64_t seed, int n_systems,
+                                 float* losses) {
+  SYMHIP_CHECK(gnn_.ready(), "learned preconditioner unavailable");
+  SYMHIP_CHECK(!logged_S_.empty(), "no logged systems to train on");
+  const size_t nsys = n_systems > 0 ? std::min<size_t>(n_systems, logged_S_.size())
+                                    : logged_S_.size();
+  SYMHIP_CHECK(steps >= 1 && P >= 1 && P <= 7 && lr > 0, "bad training arguments");
+  hipStream_t s = nullptr;
+  const int64_t n9 = (int64_t)n_nodes_ * nd_;
+  probe_x_.resize((size_t)n9 * P);
+  probe_y_.resize((size_t)n9 * P);
+  for (int k = 0; k < steps; ++k) {
+    const DBuf<double>& S = logged_S_[k % nsys];
+    LearnedPrecond::probes(probe_x_.get(), n9 * P, seed * 1000003ull + (uint64_t)k, s);
+    for (int p = 0; p < P; ++p)
+      pcg_.multiply_with(S.get(), probe_x_.get() + p * n9, probe_y_.get() + p * n9, s);
+    const float loss = gnn_.loss_grad(S.get(), probe_x_.get(), probe_y_.get(), P, s);
+    if (losses) losses[k] = loss;
+    gnn_.adam(lr, k + 1, s);
+  }
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+void SchurBackend::eval_precond(int idx, double rel_tol, int max_iters, int learned, int* iters,
+                                double* step_err) {
+  SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
+  SYMHIP_CHECK(!learned || gnn_.ready(), "learned preconditioner unavailable");
+  hipStream_t s = nullptr;
+  const int64_t n9 = (int64_t)n_nodes_ * nd_;
+  const DBuf<double>& S = logged_S_[idx];
+  const double* v = logged_v_[idx].get();
+  HIP_OK(hipMemcpyAsync(pcg_.blocks(), S.get(), S.size() * sizeof(double), hipMemcpyDeviceToDevice,
+                        s));
+  if (learned)
+    gnn_.preconditioner(pcg_.blocks(), pcg_.precond(), s);
+  else
+    pcg_.build_preconditioner(nullptr, s);
+  const PcgResult res = pcg_.solve(v, y_.get(), max_iters, rel_tol, s);
+  if (iters) *iters = res.converged ? res.iterations : -res.iterations;
+  if (step_err) {
+    exact_y_.resize((size_t)n9);
+    if (nd_ == 9) panels_from_full<9>(S.get(), s); else panels_from_full<6>(S.get(), s);
+    chol_.factorize_assembled(s);
+    chol_.solve(v, exact_y_.get(), s);
+    // relative step error |y - y_exact| / |y_exact| of this evaluation call
+    std::vector<double> a(n9), b(n9);
+    y_.download(a.data(), n9, s);
+    exact_y_.download(b.data(), n9, s);
+    HIP_OK(hipStreamSynchronize(s));
+    double num = 0, den = 0;
+    for (int64_t i = 0; i < n9; ++i) {
+      num += (a[i] - b[i]) * (a[i] - b[i]);
+      den += b[i] * b[i];
+    }
+    *step_err = den > 0 ? std::sqrt(num / den) : 0.0;
+  }
 }
 
 std::unique_ptr<SystemBackend> make_schur_backend(const HostProblem& p, const SolverOptions& o,

@@ -1,0 +1,71 @@
+"""Batched persistent-kernel LM (many small robot-3D-localization problems in one launch; no
+reference counterpart: the reference runs one Optimizer per thread, opt/optimizer.h:23).
+
+Each problem's trajectory is held to the oracle's LM (the restated Iterate,
+levenberg_marquardt_solver.tcc:138-343) on the same inputs, the reference robot-3D example's
+known answer is reproduced inside a batch, and the FP32 form reaches the FP64 oracle's cost."""
+import numpy as np
+import pytest
+
+from symforce_amd import _capi as capi
+import oracle_wrap as ow
+import problem_builders as pb
+from test_optimizer_gpu import assert_same_trajectory
+
+pytestmark = pytest.mark.gpu
+
+
+def best_error(r):
+    return r["iterations"][r["best_index"]]["new_error"]
+
+
+@pytest.mark.parametrize("lam", [capi.LAMBDA_STATIC, capi.LAMBDA_DYNAMIC])
+def test_batch_matches_oracle_per_problem(lam):
+    probs = capi.gen_localization_batch(24, seed=3)
+    params = capi.default_params(lambda_update_type=lam, iterations=30)
+    res = capi.HipBatch(probs, params).solve()
+    for prob, got in zip(probs, res["problems"]):
+        ora = ow.OracleProblem(prob).optimize(params)
+        assert_same_trajectory(got, ora, rtol=1e-8)
+        np.testing.assert_allclose(got["values"], ora["values"], rtol=0,
+                                   atol=1e-8 * max(1.0, np.abs(ora["values"]).max()))
+    assert (res["solve_ms"] > 0).all() and res["total_ms"] > 0
+
+
+def test_reference_known_answer_inside_a_batch():
+    """test/symforce_examples_robot_3d_localization_test.py:49-51, with other problems around."""
+    prob, _ = pb.robot3d_problem()
+    params = capi.default_params(initial_lambda=1e4, lambda_down_factor=0.5)
+    batch = capi.gen_localization_batch(5, seed=9)[:2] + [prob] + capi.gen_localization_batch(3, seed=4)
+    got = capi.HipBatch(batch, params).solve()["problems"][2]
+    assert got["iterations"][0]["new_error"] == pytest.approx(463700.5576620833, abs=5e-7)
+    assert best_error(got) < 140
+    assert got["status"] == capi.STATUS_SUCCESS
+    assert_same_trajectory(got, ow.OracleProblem(prob).optimize(params), rtol=1e-8)
+
+
+def test_batch_agrees_with_single_problem_optimizer():
+    probs = capi.gen_localization_batch(6, seed=12)
+    params = capi.default_params(iterations=25)
+    res = capi.HipBatch(probs, params).solve()["problems"]
+    for prob, got in zip(probs, res):
+        one = capi.HipOptimizer(prob, params).optimize()
+        assert_same_trajectory(got, one, rtol=1e-9)
+
+
+def test_fp32_form_reaches_fp64_cost():
+    probs = capi.gen_localization_batch(32, seed=5)
+    params = capi.default_params(iterations=50)
+    res = capi.HipBatch(probs, params, precision=capi.PRECISION_F32_JAC).solve()["problems"]
+    for prob, got in zip(probs, res):
+        ora = ow.OracleProblem(prob).optimize(params)
+        assert got["status"] == capi.STATUS_SUCCESS
+        # FP32 Jacobians: the converged cost agrees to single-precision accuracy
+        assert best_error(got) == pytest.approx(best_error(ora), rel=1e-4)
+
+
+def test_unsupported_problems_are_rejected():
+    with pytest.raises(RuntimeError, match="matching and odometry"):
+        capi.HipBatch([capi.gen_pose_smoothing(10, 42)])
+    with pytest.raises(RuntimeError, match="1..64"):
+        capi.HipBatch([capi.gen_localization(70, 10, 0.3, 1)])

@@ -1,0 +1,69 @@
+"""Iterative form of the Schur solve: block-Jacobi PCG on the reduced camera system (no
+reference counterpart; SparseSchurSolver factorizes S directly, sparse_schur_solver.tcc:133-163).
+
+Checked against the direct path it replaces: at a tight tolerance the LM trajectory is the
+direct one; the fallback reproduces the direct solve exactly; at a loose tolerance the optimizer
+still reaches the direct path's final cost."""
+import numpy as np
+import pytest
+
+from symforce_amd import _capi as capi
+from test_optimizer_gpu import assert_same_trajectory
+
+pytestmark = pytest.mark.gpu
+
+
+def run(prob, params, **opts):
+    opt = capi.HipOptimizer(prob, params, capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
+                                                              **opts))
+    res = opt.optimize()
+    return res, opt.linear_solver_stats()
+
+
+@pytest.mark.parametrize("gen", [lambda: capi.gen_ba_snavely(20, 600, 11),
+                                 lambda: capi.gen_ba_circle(10, 300, 4)])
+def test_tight_pcg_follows_direct_trajectory(gen):
+    prob = gen()
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=25)
+    direct, _ = run(prob, params)
+    pcg, st = run(prob, params, use_pcg=2, pcg_max_iters=2000, pcg_rel_tol=1e-13)
+    assert st["solves"] > 0 and st["fallbacks"] == 0
+    # a 1e-13 relative residual moves the step by ~cond(S) * 1e-13
+    assert_same_trajectory(pcg, direct, rtol=1e-6)
+
+
+def test_fallback_is_the_direct_solve():
+    prob = capi.gen_ba_snavely(15, 400, 21)
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=15)
+    direct, _ = run(prob, params)
+    # one CG iteration never reaches 1e-12: in strict mode every solve falls back to LDL^T
+    fb, st = run(prob, params, use_pcg=2, pcg_max_iters=1, pcg_rel_tol=1e-12)
+    assert st["fallbacks"] == st["solves"] > 0
+    assert_same_trajectory(fb, direct, rtol=1e-12)
+    np.testing.assert_array_equal(fb["values"], direct["values"])
+
+
+def test_loose_pcg_reaches_direct_final_cost():
+    prob = capi.gen_ba_snavely(30, 1500, 7)
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100)
+    direct, _ = run(prob, params)
+    pcg, st = run(prob, params, use_pcg=1, pcg_max_iters=300, pcg_rel_tol=1e-4)
+    best = lambda r: min(it["new_error"] for it in r["iterations"])
+    assert best(pcg) <= best(direct) * (1 + 1e-6)
+    assert st["fallbacks"] == 0 and st["max_iterations"] <= 300
+
+
+def test_truncated_pcg_keeps_inexact_steps():
+    prob = capi.gen_ba_snavely(30, 1500, 7)
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100)
+    direct, _ = run(prob, params)
+    pcg, st = run(prob, params, use_pcg=1, pcg_max_iters=10, pcg_rel_tol=1e-8)
+    assert st["fallbacks"] == 0 and st["max_iterations"] <= 10
+    best = lambda r: min(it["new_error"] for it in r["iterations"])
+    assert best(pcg) <= best(direct) * (1 + 1e-6)
+
+
+def test_pcg_rejected_on_cholesky_path():
+    prob = capi.gen_pose_smoothing(10, 42)
+    with pytest.raises(RuntimeError, match="Schur path only"):
+        capi.HipOptimizer(prob, capi.default_params(), capi.solver_options(use_pcg=1))
