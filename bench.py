@@ -33,34 +33,14 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 METRIC = "LM iters/s and time to FP64-oracle cost (rel 1e-6), 1k-cam BA; fast vs plain"
 ORACLE_FIXTURE = REPO / "tests" / "golden" / "ba1k_oracle.json"
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, each scaled by the same
+# counters on a 512 MiB copy kernel (profiles/rcs_assemble_1k_ba_traffic.txt); None until measured
+TRAFFIC = {}
 
 
 def lm_params(iterations, early_exit=1e-6):
     return capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=iterations,
                                early_exit_min_reduction=early_exit)
-
-
-def kernel_bytes_model(prob, opt_info):
-    """Algorithmic HBM bytes (unique data each kernel must move) per launch, FP64, ND = 9."""
-    n_obs = int(prob.info["n_obs"])
-    n_pts = int(prob.info["n_points"])
-    n_cams = int(prob.info["n_cams"])
-    nd = 9
-    f8, i4 = 8, 4
-    per_obs_idx = 6 * i4 + i4 + i4  # argument offsets, residual offset, camera index
-    return {
-        # read pixel + indices per obs, point + camera values once; write residual, W, V, g
-        "lin_points": n_obs * (per_obs_idx + 2 * f8 + 2 * f8 + nd * 3 * f8)
-        + n_pts * (3 * f8 + 6 * f8 + 3 * f8 + i4) + n_cams * 10 * f8,
-        "lin_cams": n_obs * (per_obs_idx + i4 + 2 * f8 + 3 * f8) + n_cams * (nd * nd + nd) * f8,
-        # read V, damping, g, W; write L, h, Z
-        "point_elim": n_pts * (6 + 3 + 3 + 6 + 3) * f8 + 2 * n_obs * nd * 3 * f8 + n_pts * i4,
-        # read pair list and Z once, write the S blocks
-        "rcs_assemble": opt_info["pairs"] * 2 * i4 + n_obs * nd * 3 * f8
-        + opt_info["blocks"] * nd * nd * f8,
-        "reduced_rhs": n_obs * (nd * 3 * f8 + 2 * i4) + n_pts * 3 * f8 + 2 * n_cams * nd * f8,
-        "back_sub": n_obs * (nd * 3 * f8 + i4) + n_pts * (6 + 3 + 3) * f8 + n_cams * nd * f8,
-    }
 
 
 def run_gpu(args):
@@ -94,30 +74,35 @@ def run_gpu(args):
     plain = fast = None
     if not args.no_converge:
         plain = time_to_cost(prob, opts, target)
-        fast = time_to_cost(prob, fast_options(args), target)
+        if not args.no_fast:
+            fast = time_to_cost(prob, fast_options(args), target)
     # the fast form's iteration rate and phases over the same K steps
-    fopt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0),
-                             fast_options(args))
-    fopt.optimize_resident(num_iterations=args.warmup)
-    t0 = time.perf_counter()
-    fres = fopt.optimize_resident(num_iterations=args.steps)
-    fast_ms = 1e3 * (time.perf_counter() - t0) / (len(fres["iterations"]) - 1)
-    fopt.enable_timing(True)
-    fopt.optimize_resident(num_iterations=args.steps)
-    fast_phases = fopt.phase_times()
+    fast_ms, fast_phases, fast_lin = None, {}, None
+    if not args.no_fast:
+        fopt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0),
+                                 fast_options(args, switch=False))
+        fopt.optimize_resident(num_iterations=args.warmup)
+        t0 = time.perf_counter()
+        fres = fopt.optimize_resident(num_iterations=args.steps)
+        fast_ms = 1e3 * (time.perf_counter() - t0) / (len(fres["iterations"]) - 1)
+        fopt.enable_timing(True)
+        fopt.optimize_resident(num_iterations=args.steps)
+        fast_phases = fopt.phase_times()
+        fast_lin = fopt.linear_solver_stats()
     return dict(prob=prob, opt=opt, info=info, t_gen=t_gen, t_setup=t_setup,
                 ms_per_step=ms_per_step, n_iter=n_iter, phases=phases, res=res, plain=plain,
                 fast=fast, target=target, fast_ms=fast_ms, fast_phases=fast_phases,
-                fast_lin=fopt.linear_solver_stats())
+                fast_lin=fast_lin)
 
 
-def fast_options(args):
+def fast_options(args, switch=True):
     """The fast form: block-Jacobi PCG on the reduced camera system (FP64), falling back to the
     supernodal LDL^T when CG does not reach the tolerance."""
     return capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
                                ordering=capi.ORDERING_NESTED_DISSECTION,
                                use_pcg=1, pcg_max_iters=args.pcg_max_iters,
-                               pcg_rel_tol=args.pcg_tol)
+                               pcg_rel_tol=args.pcg_tol,
+                               pcg_exact_below_reduction=args.pcg_exact_below if switch else 0.0)
 
 
 def time_to_cost(prob, opts, target):
@@ -148,24 +133,29 @@ def speedup(plain, fast):
 
 
 def roofline(r):
-    prob = r["prob"]
-    model = kernel_bytes_model(prob, dict(pairs=r["info"].get("pairs", 0),
-                                          blocks=r["info"].get("blocks", 0)))
+    """The largest single kernel launch of an LM iteration is the reduced-camera-system assembly
+    (one launch; its phase timer brackets exactly that kernel with HIP events on the optimizer's
+    stream). It runs one FP64 16x16x4 MFMA per observation pair: the useful work is
+    81 entries x 3 FMAs x 2 = 486 flops per pair, and the unique HBM bytes are the pair list, Z
+    once and the S blocks out. The supernodal LDL^T phase (74 level launches) is reported beside
+    it as a phase: flops from the symbolic analysis over the phase time."""
+    info = r["info"]
     ph = r["phases"]
-    # dominant single-kernel HBM-bound phase vs the factorization (FLOP-bound)
-    cand = {k: ph[k][0] / max(1, ph[k][1]) for k in model if ph[k][1]}
-    best = max(cand, key=cand.get)
+    ms = ph["rcs_assemble"][0] / max(1, ph["rcs_assemble"][1])
+    n_obs = int(r["prob"].info["n_obs"])
+    flops = info["pairs"] * 486
+    bytes_ = info["pairs"] * 8 + n_obs * 27 * 8 + info["blocks"] * 81 * 8
+    achieved = flops / (ms * 1e-3) / 1e12
     fac_ms = ph["factor"][0] / max(1, ph["factor"][1])
-    if fac_ms > cand[best]:
-        flops = r["info"]["flops"]
-        achieved = flops / (fac_ms * 1e-3) / 1e12
-        return dict(kernel="supernodal_ldlt_factor", bound="mfma", achieved=achieved,
-                    peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                    traffic=None, ms_per_launch=fac_ms, algorithmic_flops=flops)
-    gbs = model[best] / (cand[best] * 1e-3) / 1e9
-    return dict(kernel=best, bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=gbs / HBM_PEAK_GBS, traffic=None, ms_per_launch=cand[best],
-                algorithmic_bytes=model[best])
+    fac_tf = info["flops"] / (fac_ms * 1e-3) / 1e12
+    return dict(kernel="rcs_assemble_kernel<9>", bound="mfma", achieved=achieved,
+                peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
+                traffic=TRAFFIC.get("rcs_assemble"), ms_per_launch=ms, algorithmic_flops=flops,
+                algorithmic_bytes=bytes_,
+                hbm_gbs=bytes_ / (ms * 1e-3) / 1e9,
+                ldlt_factor_phase={"ms": fac_ms, "flops": info["flops"], "tflops": fac_tf,
+                                   "frac_fp64": fac_tf / FP64_PEAK_TFLOPS,
+                                   "levels": info["levels"]})
 
 
 def batch_leg(n_problems=4096, cpu_sample=64):
@@ -220,8 +210,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
-    ap.add_argument("--pcg-tol", type=float, default=1e-3)
-    ap.add_argument("--pcg-max-iters", type=int, default=200)
+    ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
+    ap.add_argument("--pcg-tol", type=float, default=1e-2)
+    ap.add_argument("--pcg-max-iters", type=int, default=30)
+    ap.add_argument("--pcg-exact-below", type=float, default=3e-4,
+                    help="switch the fast form to the exact LDL^T below this LM reduction")
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the run-to-convergence (time-to-cost) leg, e.g. under a profiler")
     args = ap.parse_args()
@@ -254,13 +247,14 @@ def main():
         "time_to_cost": {"target_cost": r["target"], "plain": r["plain"], "fast": r["fast"],
                          "fast_form": {"solver": "schur+block_jacobi_pcg", "dtype": "f64",
                                        "pcg_rel_tol": args.pcg_tol,
-                                       "pcg_max_iters": args.pcg_max_iters},
+                                       "pcg_max_iters": args.pcg_max_iters,
+                                       "exact_ldlt_below_reduction": args.pcg_exact_below},
                          "speedup": speedup(r["plain"], r["fast"])},
         "setup_s": r["t_setup"],
         "phases_ms_per_iter": {k: v[0] / max(1, v[1]) for k, v in r["phases"].items() if v[1]},
         "fast_ms_per_step": r["fast_ms"],
         "fast_phases_ms_per_iter": {k: v[0] / max(1, v[1]) for k, v in r["fast_phases"].items()
-                                    if v[1]},
+                                    if v[1]} or None,
         "fast_linear_solver": r["fast_lin"],
         "roofline": roof,
     }

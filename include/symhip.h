@@ -100,10 +100,13 @@ typedef struct symhip_solver_options {
   int32_t use_pcg;       /* Schur path: block-Jacobi preconditioned CG on the reduced camera
                             system. 0 off (direct LDL^T), 1 truncated (an iterate that missed
                             pcg_rel_tol within pcg_max_iters is used; LDL^T only when CG
-                            stalls), 2 strict (LDL^T whenever the tolerance is missed) */
+                            breaks down), 2 strict (LDL^T whenever the tolerance is missed) */
   int32_t pcg_max_iters;
   double pcg_rel_tol;
   int32_t pcg_preconditioner; /* SYMHIP_PRECOND_BLOCK_JACOBI or SYMHIP_PRECOND_LEARNED */
+  /* Truncated PCG only: once an accepted LM step's relative reduction falls below this value the
+     remaining iterations of the run use the direct LDL^T (0: never). */
+  double pcg_exact_below_reduction;
 } symhip_solver_options;
 
 enum symhip_pcg_preconditioner {

@@ -14,9 +14,13 @@ The definition the device kernels implement, in numpy (FP64 except where stated)
 * Three message-passing layers h' = relu([h, A h] W + b) (widths 16 -> 64 -> 64 -> 64) and a
   linear head o = h W_o + b_o (48 outputs, 45 used). Every GEMM rounds both operands to bfloat16
   (round to nearest even) and accumulates in FP32; aggregation and bias adds are FP32.
-* Per node: U = I + 0.1 * lowtri(o), L = chol(S_aa), M_a = L^-T U U^T L^-1 (so o = 0 gives the
-  block-Jacobi inverse S_aa^-1).
-* Loss over probe vectors X (N x P): sum_a |M_a (S X)_a - X_a|^2 / |X|^2.
+* Per node: with t = 2 tanh(o / 20) (bounded, slope 0.1 at 0), U lower triangular with
+  U_rr = exp(t_rr) and U_rc = t_rc (c < r),
+  L = chol(S_aa), M_a = L^-T U U^T L^-1: symmetric positive definite for any o, and o = 0 gives
+  the block-Jacobi inverse S_aa^-1.
+* Loss over probe vectors X (N x P), U = M S X: L = min_c |c U - X|^2 / |X|^2
+  = 1 - <U, X>^2 / (|U|^2 |X|^2). It is invariant to the scale of M (as CG is) and measures the
+  spread of the eigenvalues of M S seen by the probes (1 - mean^2 / mean-square).
 """
 from __future__ import annotations
 
@@ -160,9 +164,10 @@ def precond_blocks(sys, o):
     for a in range(n):
         L = np.linalg.cholesky(sys.blocks[sys.diag[a]])
         Li = np.linalg.inv(L)
-        Ua = np.eye(ND)
+        Ua = np.zeros((ND, ND))
         for k, (r, c) in enumerate(TRI):
-            Ua[r, c] += SCALE * float(o[a, k])
+            t = 2.0 * np.tanh(float(o[a, k]) / 20.0)
+            Ua[r, c] = np.exp(t) if r == c else t
         K = Ua.T @ Li
         M[a] = K.T @ K
         Linv[a], U[a] = Li, Ua
@@ -175,14 +180,18 @@ def loss_and_grad(p, sys, x, w, X):
     M, Linv, U = precond_blocks(sys, o)
     Y = sys.matvec(X).reshape(sys.n, ND, -1)
     Xs = X.reshape(sys.n, ND, -1)
-    E = np.einsum("aij,ajp->aip", M, Y) - Xs
+    Uy = np.einsum("aij,ajp->aip", M, Y)
     Z = float(np.sum(X * X))
-    loss = float(np.sum(E * E)) / Z
-    G = 2.0 / Z * np.einsum("aip,ajp->aij", E, Y)
+    A = float(np.sum(Uy * Xs))
+    B = float(np.sum(Uy * Uy))
+    loss = 1.0 - A * A / (B * Z)
+    E = (A / B) * Uy - Xs
+    G = 2.0 * A / (B * Z) * np.einsum("aip,ajp->aij", E, Y)
     dU = np.einsum("aij,ajk,alk,alm->aim", Linv, G + np.transpose(G, (0, 2, 1)), Linv, U)
     do = np.zeros_like(o, dtype=np.float32)
     for k, (r, cc) in enumerate(TRI):
-        do[:, k] = SCALE * dU[:, r, cc]
+        dt = 0.1 * (1.0 - np.tanh(o[:, k].astype(np.float64) / 20.0) ** 2)
+        do[:, k] = dt * dU[:, r, cc] * (U[:, r, r] if r == cc else 1.0)
     g = {}
     g["Wo"] = mm(c["h3"].T, do)
     g["bo"] = do.sum(axis=0).astype(np.float32)

@@ -19,9 +19,15 @@ class SupernodalLDLT {
   // Assembles A (device values in the CSC order given to setup) into panels and factorizes.
   void factorize(const double* a_values, hipStream_t s);
   // Panels already assembled by the caller (e.g. the Schur path writes them directly).
+  // The per-level launch sequence is captured in a hipGraph on the first call and replayed.
   void factorize_assembled(hipStream_t s);
   // x = A^-1 b for device vectors in the original (unpermuted) order; x may alias b.
+  // Captured per (b, x) pair and replayed.
   void solve(const double* b, double* x, hipStream_t s);
+  SupernodalLDLT() = default;
+  SupernodalLDLT(const SupernodalLDLT&) = delete;
+  SupernodalLDLT& operator=(const SupernodalLDLT&) = delete;
+  ~SupernodalLDLT();
 
   double* panels() const { return panel_.get(); }
   const CholSymbolic& symbolic() const { return sym_; }
@@ -30,6 +36,15 @@ class SupernodalLDLT {
   int64_t assembly_nnz() const { return (int64_t)asm_map_.size(); }
 
  private:
+  void launch_factorize(hipStream_t s);
+  void launch_solve(const double* b, double* x, hipStream_t s);
+  hipGraphExec_t fact_graph_ = nullptr;
+  struct SolveGraph {
+    const double* b;
+    double* x;
+    hipGraphExec_t exec;
+  };
+  std::vector<SolveGraph> solve_graphs_;  // one per (b, x) pair seen (a few state blocks)
   CholSymbolic sym_;
   DBuf<double> panel_, l11_, D_, y_;
   DBuf<int64_t> asm_map_, scat_col_;

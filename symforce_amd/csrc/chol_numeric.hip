@@ -536,7 +536,47 @@ void SupernodalLDLT::factorize(const double* a_values, hipStream_t s) {
   factorize_assembled(s);
 }
 
+SupernodalLDLT::~SupernodalLDLT() {
+  if (fact_graph_) (void)hipGraphExecDestroy(fact_graph_);
+  for (auto& g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
+}
+
+namespace {
+template <typename F>
+hipGraphExec_t capture(hipStream_t s, F&& launches) {
+  hipGraph_t g;
+  hipGraphExec_t exec;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launches();
+  HIP_OK(hipStreamEndCapture(s, &g));
+  HIP_OK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+  return exec;
+}
+}  // namespace
+
 void SupernodalLDLT::factorize_assembled(hipStream_t s) {
+  if (sym_.n_levels == 0) return;
+  if (!fact_graph_) fact_graph_ = capture(s, [&] { launch_factorize(s); });
+  HIP_OK(hipGraphLaunch(fact_graph_, s));
+}
+
+void SupernodalLDLT::solve(const double* b, double* x, hipStream_t s) {
+  if (sym_.n == 0) return;
+  for (const auto& g : solve_graphs_)
+    if (g.b == b && g.x == x) {
+      HIP_OK(hipGraphLaunch(g.exec, s));
+      return;
+    }
+  if (solve_graphs_.size() >= 8) {  // bound the cache; the oldest goes
+    (void)hipGraphExecDestroy(solve_graphs_.front().exec);
+    solve_graphs_.erase(solve_graphs_.begin());
+  }
+  solve_graphs_.push_back({b, x, capture(s, [&] { launch_solve(b, x, s); })});
+  HIP_OK(hipGraphLaunch(solve_graphs_.back().exec, s));
+}
+
+void SupernodalLDLT::launch_factorize(hipStream_t s) {
   const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
     const int nt = lvl_trsm_ptr_[l + 1] - lvl_trsm_ptr_[l];
@@ -553,9 +593,8 @@ void SupernodalLDLT::factorize_assembled(hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
-void SupernodalLDLT::solve(const double* b, double* x, hipStream_t s) {
+void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
   const int n = sym_.n;
-  if (n == 0) return;
   const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(), b,
                      y_.get());

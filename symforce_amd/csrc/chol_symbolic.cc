@@ -184,10 +184,25 @@ struct Dissector {
     }
     if (best < 0) best = median;
     std::vector<int32_t> A, B, sep;
+    int64_t wa = 0, wb = 0, ws = 0;
     for (int32_t v : q) {
-      if (dist[v] < best) A.push_back(v);
-      else if (dist[v] > best) B.push_back(v);
-      else sep.push_back(v);
+      if (dist[v] < best) {
+        A.push_back(v);
+        wa += weight[v];
+      } else if (dist[v] > best) {
+        B.push_back(v);
+        wb += weight[v];
+      } else {
+        sep.push_back(v);
+        ws += weight[v];
+      }
+    }
+    // A level separator heavier than the smaller part (e.g. the landmarks of a bipartite
+    // camera/landmark graph) would be eliminated last and fill in densely: order this subgraph
+    // by minimum degree instead.
+    if (ws > std::min(wa, wb)) {
+      leaf_order(verts);
+      return;
     }
     dissect(std::move(A));
     dissect(std::move(B));

@@ -55,7 +55,8 @@ std::vector<int32_t> min_degree_order(const std::vector<std::vector<int32_t>>& a
 // Nested-dissection ordering of a node graph (the family METIS belongs to; the reference orders
 // with METIS_NodeND through Eigen::MetisOrdering): recursive vertex separators taken from a BFS
 // level structure rooted at a pseudo-peripheral node, parts first, separator last; subgraphs of
-// at most leaf_nodes nodes are ordered by minimum degree. The shallower elimination tree gives the
+// at most leaf_nodes nodes, and subgraphs whose level separator outweighs the smaller part, are
+// ordered by minimum degree. The shallower elimination tree gives the
 // level-scheduled numeric factorization more independent supernodes per level.
 // Maps a symhip_ordering code to the enum (throws on an unknown code).
 Ordering ordering_from_code(int32_t code);

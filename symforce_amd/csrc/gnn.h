@@ -1,8 +1,11 @@
 // Learned block-Jacobi correction for PCG on the reduced camera system: a small message-passing
-// network over the camera covisibility graph predicts, per camera, a lower-triangular T_a; the
-// preconditioner block is M_a = L_a^-T (I + T_a)(I + T_a)^T L_a^-1 with S_aa = L_a L_a^T (so
-// T = 0 is exactly block Jacobi). Training minimizes sum_a |M_a (S X)_a - X_a|^2 / |X|^2 over
-// random probe vectors X on Schur systems logged from the optimizer's own solves. All GEMMs are
+// network over the camera covisibility graph predicts, per camera, a lower-triangular U_a with a
+// positive (exponential) diagonal, entries bounded through 2 tanh(o / 20); the preconditioner block is M_a = L_a^-T U_a U_a^T L_a^-1
+// with S_aa = L_a L_a^T, symmetric positive definite for any output, and U = I (zero output) is
+// exactly block Jacobi. Training minimizes the scale-invariant probe loss
+// min_c |c M S X - X|^2 / |X|^2 = 1 - <M S X, X>^2 / (|M S X|^2 |X|^2) (the spread of the
+// eigenvalues of M S seen by random probes X) on Schur systems logged from the optimizer's own
+// solves. All GEMMs are
 // BF16 matrix-core products with FP32 accumulation; the per-camera algebra is FP64.
 #pragma once
 

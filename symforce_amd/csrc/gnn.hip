@@ -197,17 +197,21 @@ __global__ void relu_mask_kernel(int64_t n, const float* __restrict__ pre,
   if (t < n) out[t] = pre[t] > 0 ? g[t] : 0.0f;
 }
 
-// Per camera (one wavefront, FP64 in LDS): L = chol(S_aa), Li = L^-1, U = I + 0.1 lowtri(o),
-// K = U^T Li, M = K^T K -> Minv. With probes: E = M Y_a - X_a, loss part sum E^2 / Z,
-// G = 2 E Y_a^T / Z, dU = Li (G + G^T) Li^T U, dout = 0.1 lowtri(dU).
+// Per camera (one wavefront, FP64 in LDS): L = chol(S_aa), Li = L^-1, t = 2 tanh(o / 20),
+// U = lowtri(t) with exp on the diagonal, K = U^T Li, M = K^T K -> Minv. With probes (Y = S X), pass 1 (AB null)
+// writes the camera's parts of A = <M Y, X> and B = |M Y|^2; pass 2 (AB = {A, B}) forms
+// E = (A / B) M Y_a - X_a, G = 2 A / (B Z) E Y_a^T, dU = Li (G + G^T) Li^T U and
+// dout = 0.1 lowtri(dU) (times U_rr on the exponential diagonal).
 __global__ void __launch_bounds__(64) head_kernel(int32_t n, const int32_t* __restrict__ diag,
                                                   const double* __restrict__ S,
                                                   const float* __restrict__ o,
                                                   double* __restrict__ Minv,
                                                   const double* __restrict__ X,
                                                   const double* __restrict__ Y, int P, double invZ,
+                                                  const double* __restrict__ AB,
                                                   float* __restrict__ dout,
-                                                  double* __restrict__ loss_part) {
+                                                  double* __restrict__ partA,
+                                                  double* __restrict__ partB) {
   __shared__ double L[kNd][kNd + 1], Li[kNd][kNd + 1], U[kNd][kNd + 1], Km[kNd][kNd + 1],
       Mm[kNd][kNd + 1], E[kNd][8], G[kNd][kNd + 1], T[kNd][kNd + 1];
   const int a = blockIdx.x, lane = threadIdx.x;
@@ -215,7 +219,8 @@ __global__ void __launch_bounds__(64) head_kernel(int32_t n, const int32_t* __re
   for (int e = lane; e < kNe; e += 64) {
     const int r = e / kNd, c = e % kNd;
     L[r][c] = c <= r ? S[(int64_t)diag[a] * kNe + e] : 0.0;
-    U[r][c] = (r == c ? 1.0 : 0.0) + (c <= r ? kScale * (double)o[(int64_t)a * kO + tri(r, c)] : 0.0);
+    const double oc = c <= r ? 2.0 * tanh((double)o[(int64_t)a * kO + tri(r, c)] / 20.0) : 0.0;
+    U[r][c] = r == c ? exp(oc) : oc;
   }
   __syncthreads();
   for (int k = 0; k < kNd; ++k) {
@@ -255,47 +260,58 @@ __global__ void __launch_bounds__(64) head_kernel(int32_t n, const int32_t* __re
   }
   if (!X) return;
   __syncthreads();
-  double lp = 0;
-  if (lane < kNd * P) {
-    const int i = lane % kNd, p = lane / kNd;
-    double s = -X[p * n9 + (int64_t)a * kNd + i];
-    for (int k = 0; k < kNd; ++k) s += Mm[i][k] * Y[p * n9 + (int64_t)a * kNd + k];
-    E[i][p] = s;
-    lp = s * s;
+  // u = M Y_a (lane per (row, probe))
+  double u = 0, xv = 0;
+  const bool act = lane < kNd * P;
+  const int i = lane % kNd, p = lane / kNd;
+  if (act) {
+    xv = X[p * n9 + (int64_t)a * kNd + i];
+    for (int k = 0; k < kNd; ++k) u += Mm[i][k] * Y[p * n9 + (int64_t)a * kNd + k];
   }
-  lp = wsum(lp);
-  if (lane == 0) loss_part[a] = lp * invZ;
+  if (!AB) {
+    const double pa = wsum(act ? u * xv : 0.0), pb = wsum(act ? u * u : 0.0);
+    if (lane == 0) {
+      partA[a] = pa;
+      partB[a] = pb;
+    }
+    return;
+  }
+  const double A = AB[0], B = AB[1];
+  const double c = A / B, scale = 2.0 * A / B * invZ;
+  if (act) E[i][p] = c * u - xv;
   __syncthreads();
   for (int e = lane; e < kNe; e += 64) {
-    const int r = e / kNd, c = e % kNd;
+    const int r = e / kNd, cc = e % kNd;
     double s = 0;
-    for (int p = 0; p < P; ++p) s += E[r][p] * Y[p * n9 + (int64_t)a * kNd + c];
-    G[r][c] = 2.0 * invZ * s;
+    for (int q = 0; q < P; ++q) s += E[r][q] * Y[q * n9 + (int64_t)a * kNd + cc];
+    G[r][cc] = scale * s;
   }
   __syncthreads();
   // T = Li (G + G^T)
   for (int e = lane; e < kNe; e += 64) {
-    const int r = e / kNd, c = e % kNd;
+    const int r = e / kNd, cc = e % kNd;
     double s = 0;
-    for (int k = 0; k < kNd; ++k) s += Li[r][k] * (G[k][c] + G[c][k]);
-    T[r][c] = s;
+    for (int k = 0; k < kNd; ++k) s += Li[r][k] * (G[k][cc] + G[cc][k]);
+    T[r][cc] = s;
   }
   __syncthreads();
   // Km <- T Li^T
   for (int e = lane; e < kNe; e += 64) {
-    const int r = e / kNd, c = e % kNd;
+    const int r = e / kNd, cc = e % kNd;
     double s = 0;
-    for (int k = 0; k < kNd; ++k) s += T[r][k] * Li[c][k];
-    Km[r][c] = s;
+    for (int k = 0; k < kNd; ++k) s += T[r][k] * Li[cc][k];
+    Km[r][cc] = s;
   }
   __syncthreads();
   // dU = Km U, packed lower part into dout
   for (int e = lane; e < kNe; e += 64) {
-    const int r = e / kNd, c = e % kNd;
-    if (c > r) continue;
+    const int r = e / kNd, cc = e % kNd;
+    if (cc > r) continue;
     double s = 0;
-    for (int k = 0; k < kNd; ++k) s += Km[r][k] * U[k][c];
-    dout[(int64_t)a * kO + tri(r, c)] = (float)(kScale * s);
+    for (int k = 0; k < kNd; ++k) s += Km[r][k] * U[k][cc];
+    const double th = tanh((double)o[(int64_t)a * kO + tri(r, cc)] / 20.0);
+    dout[(int64_t)a * kO + tri(r, cc)] =
+        (float)(kScale * (1.0 - th * th) * s * (r == cc ? U[r][r] : 1.0));
   }
   if (lane < kO - 45) dout[(int64_t)a * kO + 45 + lane] = 0.0f;
 }
@@ -376,7 +392,7 @@ void LearnedPrecond::setup(int32_t n_nodes, const std::vector<int32_t>& row_ptr,
   dz_.resize(n * 2 * kH);
   o_.resize(n * kO);
   do_.resize(n * kO);
-  part_.resize(n + 1);
+  part_.resize(2 * n + 4);
   red_.resize(kReduceBlocks);
   params_.resize(kParams);
   grad_.resize(kParams);
@@ -438,7 +454,7 @@ void LearnedPrecond::preconditioner(const double* S, double* Minv, hipStream_t s
   SYMHIP_CHECK(ready(), "learned preconditioner not set up");
   forward(S, s);
   hipLaunchKernelGGL(head_kernel, dim3(n_), dim3(64), 0, s, n_, diag_.get(), S, o_.get(), Minv,
-                     nullptr, nullptr, 0, 0.0, nullptr, nullptr);
+                     nullptr, nullptr, 0, 0.0, nullptr, nullptr, nullptr, nullptr);
   HIP_OK(hipGetLastError());
 }
 
@@ -449,14 +465,22 @@ float LearnedPrecond::loss_grad(const double* S, const double* X, const double* 
   const int n = n_;
   const int64_t n9 = (int64_t)n * kNd;
   // Z = |X|^2 on the host side of the boundary: X is small (P n 9 doubles)
-  dot(X, X, n9 * P, red_.get(), part_.get() + n, s);
+  dot(X, X, n9 * P, red_.get(), part_.get() + 2 * n + 2, s);
   double Z = 0;
-  HIP_OK(hipMemcpyAsync(&Z, part_.get() + n, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(&Z, part_.get() + 2 * n + 2, sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   SYMHIP_CHECK(Z > 0, "learned preconditioner: zero probes");
   forward(S, s);
+  // pass 1: parts of A = <M Y, X>, B = |M Y|^2; reduce; pass 2: gradient at the optimal scale
+  double* pA = part_.get();
+  double* pB = part_.get() + n;
+  double* AB = part_.get() + 2 * n;
   hipLaunchKernelGGL(head_kernel, dim3(n), dim3(64), 0, s, n, diag_.get(), S, o_.get(), nullptr,
-                     X, Y, P, 1.0 / Z, do_.get(), part_.get());
+                     X, Y, P, 1.0 / Z, nullptr, nullptr, pA, pB);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, s, n, pA, AB);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, s, n, pB, AB + 1);
+  hipLaunchKernelGGL(head_kernel, dim3(n), dim3(64), 0, s, n, diag_.get(), S, o_.get(), nullptr,
+                     X, Y, P, 1.0 / Z, AB, do_.get(), nullptr, nullptr);
   float* Gp = grad_.get();
   const float* Pp = params_.get();
   // head
@@ -483,12 +507,11 @@ float LearnedPrecond::loss_grad(const double* S, const double* X, const double* 
                          col_.get(), mirror_.get(), w_.get(), dz_.get(), kH, dh_.get());
     }
   }
-  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, s, n, part_.get(),
-                     part_.get() + n);
-  double loss = 0;
-  HIP_OK(hipMemcpyAsync(&loss, part_.get() + n, sizeof(double), hipMemcpyDeviceToHost, s));
+  double ab[2] = {0, 0};
+  HIP_OK(hipMemcpyAsync(ab, AB, sizeof(ab), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   HIP_OK(hipGetLastError());
+  const double loss = ab[1] > 0 ? 1.0 - ab[0] * ab[0] / (ab[1] * Z) : 1.0;
   return (float)loss;
 }
 

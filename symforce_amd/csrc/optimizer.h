@@ -22,6 +22,7 @@ struct SolverOptions {
   int pcg_max_iters = 100;
   double pcg_rel_tol = 1e-6;
   int pcg_preconditioner = 0;  // 0 block Jacobi, 1 learned correction
+  double pcg_exact_below_reduction = 0;
 };
 
 // A linear system backend owns the linearization storage of the three LM state blocks and the
@@ -52,6 +53,9 @@ class SystemBackend {
   // Iterative linear solver statistics: out[0]=solves, out[1]=total iterations, out[2]=fallbacks
   // to the direct factorization, out[3]=most iterations in one solve (zeros for direct solvers).
   virtual void linear_stats(int64_t* out4) const { out4[0] = out4[1] = out4[2] = out4[3] = 0; }
+  // Called at the start of every optimize run and after every LM iteration record.
+  virtual void begin_run() {}
+  virtual void observe(const symhip_iteration_stats& rec) { (void)rec; }
   // Learned PCG preconditioner (Schur path with PCG only; the defaults reject the call).
   // Logging keeps copies of the next max reduced camera systems (full block storage) and their
   // right-hand sides as they are assembled.

@@ -347,6 +347,7 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
   valid_[init_] = valid_[free_] = false;
   hipLaunchKernelGGL(lm_reset_kernel, dim3(1), dim3(1), 0, stream_, dstate_, p_.initial_lambda,
                      p_.dynamic_lambda_update_beta);
+  backend_->begin_run();
   iters.clear();
   iter_times_.clear();
   int32_t best_index = 0;
@@ -438,6 +439,7 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   timer_.collect();
   HIP_OK(hipGetLastError());
   iters.push_back(hstate_->rec);
+  backend_->observe(hstate_->rec);
   if (!hstate_->accept) {
     std::swap(new_, init_);  // SwapNewAndInit
   } else {

@@ -30,8 +30,10 @@ class BlockPcg {
   void build_preconditioner(const double* correction, hipStream_t s);
   // x = S^-1 b to ||r|| <= rel_tol ||b|| or max_iters. b, x: device, n * nd. Synchronizes the
   // stream every check_every iterations (and at the end) to read the convergence flag.
+  // Iterations run in captured chunks of kChunk; the host reads the stop flag between chunks.
   PcgResult solve(const double* b, double* x, int max_iters, double rel_tol, hipStream_t s,
-                  int check_every = 16);
+                  int check_every = kChunk);
+  static constexpr int kChunk = 16;  // even: a chunk ends on slot 1 and the next starts on 0
   // y = S x (device vectors), for checks and step-error measurement.
   void multiply(const double* x, double* y, hipStream_t s) { multiply_with(S_.get(), x, y, s); }
   // y = S x for other blocks S of the same pattern.
@@ -45,6 +47,11 @@ class BlockPcg {
   DBuf<int32_t> row_ptr_, col_, diag_;
   DBuf<double> S_, Minv_, r_, z_, p_[2], q_, partA_, partB_, scal_;
   double* host_scal_ = nullptr;  // pinned
+  hipGraphExec_t chunk_ = nullptr;  // kChunk iterations, captured for (x, max_iters, tol)
+  double* chunk_x_ = nullptr;
+  int chunk_max_ = -1;
+  double chunk_tol2_ = -1;
+  void capture_chunk(double* x, int max_iters, double tol2, hipStream_t s);
  public:
   BlockPcg() = default;
   BlockPcg(const BlockPcg&) = delete;

@@ -8,7 +8,10 @@
 // those partials, takes alpha = rz_k / p.q, updates x, r, applies the block-Jacobi
 // preconditioner z = M r and writes the next partial sums. Scalars live in a small device array:
 //   [0], [1] rz by iteration parity, [2] rr_0, [3] state (0 running, 1 converged,
-//   2 breakdown), [4] iteration count at the stop, [5] last rr.
+//   2 breakdown), [4] iteration count at the stop, [5] last rr, [6], [7] the iteration index k
+//   by slot. The index is read from the device (kernel s of an iteration pair uses slot s and
+//   the spmv kernel writes k + 1 into the other slot), so a chunk of iterations is the same
+//   launch sequence every time and is captured once in a hipGraph and replayed.
 #include <algorithm>
 
 #include "./pcg.h"
@@ -166,11 +169,14 @@ __device__ __forceinline__ double row_times(const int32_t* __restrict__ row_ptr,
 
 template <int ND>
 __global__ void __launch_bounds__(256) pcg_spmv_kernel(
-    int k, int32_t n, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-    const double* __restrict__ S, const double* __restrict__ z, const double* __restrict__ p_old,
-    double* __restrict__ p_new, double* __restrict__ q, const double* __restrict__ partB,
-    int nbB, double* __restrict__ partA, double* __restrict__ scal, double tol2) {
+    int slot, int max_iters, int32_t n, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ col, const double* __restrict__ S, const double* __restrict__ z,
+    const double* __restrict__ p_old, double* __restrict__ p_new, double* __restrict__ q,
+    const double* __restrict__ partB, int nbB, double* __restrict__ partA,
+    double* __restrict__ scal, double tol2) {
   if (scal[3] != 0.0) return;
+  const int k = (int)scal[6 + slot];
+  if (k > max_iters) return;
   const int lane = threadIdx.x & 63;
   const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   double rz = 0, rr = 0;
@@ -188,12 +194,14 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(
     scal[k & 1] = rz;
     if (k == 0) scal[2] = rr;
     scal[5] = rr;
+    scal[6 + (slot ^ 1)] = k + 1;
     if (conv || !finite) {
       scal[3] = conv ? 1.0 : 2.0;
       scal[4] = k;
     }
   }
-  if (conv || !finite || a >= n) return;
+  // k == max_iters only checks the last residual
+  if (conv || !finite || a >= n || k == max_iters) return;
   const double qi = row_times<ND>(row_ptr, col, S, z, p_old, beta, a, lane);
   double pq = 0;
   if (lane < ND) {
@@ -209,11 +217,13 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(
 
 template <int ND>
 __global__ void __launch_bounds__(256) pcg_update_kernel(
-    int k, int32_t n, const double* __restrict__ Minv, const double* __restrict__ p_new,
-    const double* __restrict__ q, const double* __restrict__ partA, double* __restrict__ x,
-    double* __restrict__ r, double* __restrict__ z, double* __restrict__ partB,
-    double* __restrict__ scal) {
+    int slot, int max_iters, int32_t n, const double* __restrict__ Minv,
+    const double* __restrict__ p_new, const double* __restrict__ q,
+    const double* __restrict__ partA, double* __restrict__ x, double* __restrict__ r,
+    double* __restrict__ z, double* __restrict__ partB, double* __restrict__ scal) {
   if (scal[3] != 0.0) return;
+  const int k = (int)scal[6 + slot];
+  if (k >= max_iters) return;
   constexpr int RPB = 256 / ND, NE = ND * ND;
   __shared__ double rs[RPB * ND];
   double pq = 0, dummy = 0;
@@ -277,6 +287,7 @@ void dispatch_nd(int nd, F&& f) {
 }  // namespace
 
 BlockPcg::~BlockPcg() {
+  if (chunk_) (void)hipGraphExecDestroy(chunk_);
   if (host_scal_) (void)hipHostFree(host_scal_);
 }
 
@@ -333,6 +344,32 @@ void BlockPcg::multiply_with(const double* S, const double* x, double* y, hipStr
   HIP_OK(hipGetLastError());
 }
 
+void BlockPcg::capture_chunk(double* x, int max_iters, double tol2, hipStream_t s) {
+  if (chunk_ && chunk_x_ == x && chunk_max_ == max_iters && chunk_tol2_ == tol2) return;
+  if (chunk_) (void)hipGraphExecDestroy(chunk_);
+  chunk_ = nullptr;
+  hipGraph_t g;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  dispatch_nd(nd_, [&]<int ND>() {
+    for (int i = 0; i < kChunk; ++i) {
+      const int slot = i & 1;
+      hipLaunchKernelGGL((pcg_spmv_kernel<ND>), dim3(grid_for(n_, 4)), dim3(256), 0, s, slot,
+                         max_iters, n_, row_ptr_.get(), col_.get(), S_.get(), z_.get(),
+                         p_[slot ^ 1].get(), p_[slot].get(), q_.get(), partB_.get(),
+                         n_upd_blocks_, partA_.get(), scal_.get(), tol2);
+      hipLaunchKernelGGL((pcg_update_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, slot,
+                         max_iters, n_, Minv_.get(), p_[slot].get(), q_.get(), partA_.get(), x,
+                         r_.get(), z_.get(), partB_.get(), scal_.get());
+    }
+  });
+  HIP_OK(hipStreamEndCapture(s, &g));
+  HIP_OK(hipGraphInstantiate(&chunk_, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+  chunk_x_ = x;
+  chunk_max_ = max_iters;
+  chunk_tol2_ = tol2;
+}
+
 PcgResult BlockPcg::solve(const double* b, double* x, int max_iters, double rel_tol,
                           hipStream_t s, int check_every) {
   PcgResult res;
@@ -340,34 +377,26 @@ PcgResult BlockPcg::solve(const double* b, double* x, int max_iters, double rel_
     res.converged = true;
     return res;
   }
-  SYMHIP_CHECK(max_iters >= 1 && check_every >= 1, "BlockPcg: bad iteration limits");
+  SYMHIP_CHECK(max_iters >= 1, "BlockPcg: bad iteration limit");
+  (void)check_every;
   const double tol2 = rel_tol * rel_tol;
+  capture_chunk(x, max_iters, tol2, s);
   dispatch_nd(nd_, [&]<int ND>() {
     hipLaunchKernelGGL((pcg_init_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, n_, b,
                        Minv_.get(), x, r_.get(), z_.get(), p_[1].get(), partB_.get(), scal_.get());
-    auto spmv = [&](int k) {
-      hipLaunchKernelGGL((pcg_spmv_kernel<ND>), dim3(grid_for(n_, 4)), dim3(256), 0, s, k, n_,
-                         row_ptr_.get(), col_.get(), S_.get(), z_.get(), p_[(k + 1) & 1].get(),
-                         p_[k & 1].get(), q_.get(), partB_.get(), n_upd_blocks_, partA_.get(),
-                         scal_.get(), tol2);
-    };
-    int k = 0;
-    for (; k < max_iters; ++k) {
-      spmv(k);
-      hipLaunchKernelGGL((pcg_update_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, k, n_,
-                         Minv_.get(), p_[k & 1].get(), q_.get(), partA_.get(), x, r_.get(),
-                         z_.get(), partB_.get(), scal_.get());
-      if ((k + 1) % check_every == 0 && k + 1 < max_iters) {
-        HIP_OK(hipMemcpyAsync(host_scal_, scal_.get(), kScal * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (host_scal_[3] != 0.0) break;
-      }
-    }
-    // the check of the last residual (a no-op when already stopped)
-    spmv(max_iters);
   });
   HIP_OK(hipGetLastError());
+  // chunks of kChunk iterations cover k = 0 .. max_iters (the last spmv only checks)
+  const int chunks = max_iters / kChunk + 1;
+  for (int c = 0; c < chunks; ++c) {
+    HIP_OK(hipGraphLaunch(chunk_, s));
+    if (c + 1 < chunks) {
+      HIP_OK(hipMemcpyAsync(host_scal_, scal_.get(), kScal * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      if (host_scal_[3] != 0.0) break;
+    }
+  }
   HIP_OK(hipMemcpyAsync(host_scal_, scal_.get(), kScal * sizeof(double), hipMemcpyDeviceToHost,
                         s));
   HIP_OK(hipStreamSynchronize(s));

@@ -477,6 +477,12 @@ class SchurBackend final : public SystemBackend {
     out[6] = n_pairs_;
     out[7] = n_obs_;
   }
+  void begin_run() override { exact_now_ = false; }
+  void observe(const symhip_iteration_stats& rec) override {
+    if (use_pcg_ && exact_below_ > 0 && rec.update_accepted && rec.relative_reduction >= 0 &&
+        rec.relative_reduction < exact_below_)
+      exact_now_ = true;
+  }
   void log_systems(int max) override;
   int logged_systems() const override { return (int)logged_S_.size(); }
   int rcs_pattern(int32_t* row_ptr, int32_t* col) const override;
@@ -501,6 +507,7 @@ class SchurBackend final : public SystemBackend {
   void relin_nd(int b, const double* values, hipStream_t s);
 
   const HostProblem* host_;
+  hipStream_t stream_;  // the optimizer's stream (the training / evaluation calls use it too)
   int nd_ = 0, ftype_ = 0;
   int32_t n_nodes_ = 0, n_points_ = 0, n_obs_ = 0, n_blocks_ = 0, N_ = 0, M_ = 0;
   int64_t n_pairs_ = 0;
@@ -524,7 +531,8 @@ class SchurBackend final : public SystemBackend {
   std::unique_ptr<CscLinearizer> csc_;  // reference-layout H for best-linearization output
   // Iterative form: PCG on the reduced camera system in full block storage, falling back to the
   // supernodal LDL^T when it does not reach the tolerance.
-  bool use_pcg_ = false, pcg_strict_ = false;
+  bool use_pcg_ = false, pcg_strict_ = false, exact_now_ = false;
+  double exact_below_ = 0;
   int pcg_max_iters_ = 100;
   double pcg_rel_tol_ = 1e-6;
   BlockPcg pcg_;
@@ -554,7 +562,7 @@ struct UnionFind {
 }  // namespace
 
 SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStream_t s)
-    : host_(&P) {
+    : host_(&P), stream_(s) {
   const int32_t S = (int32_t)P.state_keys.size();
   N_ = P.N;
   M_ = P.M;
@@ -827,6 +835,8 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     SYMHIP_CHECK(o.use_pcg == 1 || o.use_pcg == 2, "use_pcg must be 0, 1 (truncated) or 2 (strict)");
     use_pcg_ = true;
     pcg_strict_ = o.use_pcg == 2;
+    SYMHIP_CHECK(o.pcg_exact_below_reduction >= 0, "pcg_exact_below_reduction must be >= 0");
+    exact_below_ = o.pcg_exact_below_reduction;
     pcg_max_iters_ = o.pcg_max_iters;
     pcg_rel_tol_ = o.pcg_rel_tol;
     // full symmetric block pattern: row a holds b for every lower block (a, b) and (b, a)
@@ -1003,8 +1013,8 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
                        dim3(256), 0, s, n_obs_, obs_point_.get(), Lp_.get(), W_[b].get(), Z_.get());
   }
   auto assemble = [&](bool full) {
-    PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);
     if (!full) fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
+    PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);  // exactly the one assembly kernel
     hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(grid_for((int64_t)n_blocks_ * 64, 256)),
                        dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
                        pairs_.get(), Z_.get(), Bdiag_[b].get(), Bfac_[b].get(), dnode_.get(),
@@ -1019,14 +1029,15 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
     chol_.solve(v_.get(), y_.get(), s);
   };
-  assemble(use_pcg_);
+  const bool pcg_now = use_pcg_ && !exact_now_;
+  assemble(pcg_now);
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
     hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
                        dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
                        obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get());
   }
-  if (use_pcg_ && (int)logged_S_.size() < log_max_) {
+  if (pcg_now && (int)logged_S_.size() < log_max_) {
     DBuf<double> Sc, vc;
     Sc.resize((size_t)pcg_.n_blocks() * ND * ND);
     vc.resize((size_t)n_nodes_ * ND);
@@ -1037,7 +1048,7 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     logged_S_.push_back(std::move(Sc));
     logged_v_.push_back(std::move(vc));
   }
-  if (use_pcg_) {
+  if (pcg_now) {
     PcgResult res;
     {
       PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
@@ -1051,10 +1062,11 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     pcg_iters_ += res.iterations;
     pcg_max_seen_ = std::max<int64_t>(pcg_max_seen_, res.iterations);
     // Truncated mode (use_pcg == 1) keeps a CG iterate that did not reach the tolerance: every
-    // CG iterate lowers the quadratic model, and the LM gain ratio judges the step. The direct
-    // LDL^T takes over when CG stalls (breakdown, no residual reduction) or, in strict mode
-    // (use_pcg == 2), whenever the tolerance was not reached.
-    const bool stalled = res.breakdown || !(res.rel_residual < 1.0);
+    // CG iterate lowers the quadratic model (the residual 2-norm need not fall monotonically),
+    // and the LM gain ratio judges the step. The direct LDL^T takes over when CG breaks down
+    // (S not SPD to working precision, non-finite sums) or, in strict mode (use_pcg == 2),
+    // whenever the tolerance was not reached.
+    const bool stalled = res.breakdown;
     if (stalled || (pcg_strict_ && !res.converged)) {
       ++pcg_fallbacks_;
       assemble(false);
@@ -1131,7 +1143,7 @@ int SchurBackend::rcs_pattern(int32_t* row_ptr, int32_t* col) const {
 
 void SchurBackend::logged_system(int idx, double* blocks, double* rhs) const {
   SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
-  hipStream_t s = nullptr;
+  hipStream_t s = stream_;
   if (blocks) logged_S_[idx].download(blocks, logged_S_[idx].size(), s);
   if (rhs) logged_v_[idx].download(rhs, logged_v_[idx].size(), s);
   HIP_OK(hipStreamSynchronize(s));
@@ -1140,15 +1152,15 @@ void SchurBackend::logged_system(int idx, double* blocks, double* rhs) const {
 void SchurBackend::precond_params(float* get, const float* set) {
   SYMHIP_CHECK(gnn_.ready(), "the learned preconditioner needs the Schur solver with use_pcg "
                              "and 9-dof camera nodes");
-  if (set) gnn_.set_params(set, nullptr);
-  if (get) gnn_.get_params(get, nullptr);
+  if (set) gnn_.set_params(set, stream_);
+  if (get) gnn_.get_params(get, stream_);
 }
 
 float SchurBackend::precond_loss_grad(int idx, const double* X, int P, float* grad) {
   SYMHIP_CHECK(gnn_.ready(), "learned preconditioner unavailable");
   SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
   SYMHIP_CHECK(P >= 1 && P <= 7, "1..7 probe vectors");
-  hipStream_t s = nullptr;
+  hipStream_t s = stream_;
   const int64_t n9 = (int64_t)n_nodes_ * nd_;
   probe_x_.upload(X, (size_t)n9 * P, s);
   probe_y_.resize((size_t)n9 * P);
@@ -1170,7 +1182,7 @@ void SchurBackend::train_precond(int steps, float lr, int P, uint64_t seed, int 
   const size_t nsys = n_systems > 0 ? std::min<size_t>(n_systems, logged_S_.size())
                                     : logged_S_.size();
   SYMHIP_CHECK(steps >= 1 && P >= 1 && P <= 7 && lr > 0, "bad training arguments");
-  hipStream_t s = nullptr;
+  hipStream_t s = stream_;
   const int64_t n9 = (int64_t)n_nodes_ * nd_;
   probe_x_.resize((size_t)n9 * P);
   probe_y_.resize((size_t)n9 * P);
@@ -1190,7 +1202,7 @@ void SchurBackend::eval_precond(int idx, double rel_tol, int max_iters, int lear
                                 double* step_err) {
   SYMHIP_CHECK(idx >= 0 && idx < (int)logged_S_.size(), "no such logged system");
   SYMHIP_CHECK(!learned || gnn_.ready(), "learned preconditioner unavailable");
-  hipStream_t s = nullptr;
+  hipStream_t s = stream_;
   const int64_t n9 = (int64_t)n_nodes_ * nd_;
   const DBuf<double>& S = logged_S_[idx];
   const double* v = logged_v_[idx].get();

@@ -123,5 +123,6 @@ def test_bal_problem_optimizes_like_the_oracle(tmp_path):
     hip = capi.HipOptimizer(prob, params,
                             capi.solver_options(linear_solver=capi.SOLVER_SCHUR)).optimize()
     ora = ow.OracleProblem(prob).optimize(params, solver=ow.SCHUR, c_dim=int(prob.info["c_dim"]))
-    # the free 7-dof similarity gauge leaves H damped only by lambda: ~1e-11 cost differences
-    assert_same_trajectory(hip, ora, rtol=1e-7)
+    # the free 7-dof similarity gauge leaves H damped only by lambda: ~1e-11 cost differences,
+    # which the DYNAMIC lambda update amplifies through the gain ratio
+    assert_same_trajectory(hip, ora, rtol=1e-7, lambda_rtol=1e-6)

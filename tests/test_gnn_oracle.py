@@ -74,6 +74,18 @@ def test_untrained_network_is_block_jacobi():
         np.testing.assert_allclose(M[a], np.linalg.inv(s.blocks[s.diag[a]]), rtol=1e-10, atol=1e-14)
 
 
+def test_loss_is_scale_invariant():
+    s = ring_system(seed=5)
+    x, w = G.features(s)
+    X = np.random.default_rng(2).standard_normal((s.n * 9, 3))
+    p = G.init_params(0)
+    l0 = G.loss_and_grad(p, s, x, w, X)[0]
+    s2 = G.System(s.row_ptr, s.col, s.blocks * 7.0)  # M scales by 1/7, S by 7: same M S
+    x2, w2 = G.features(s2)
+    assert G.loss_and_grad(p, s2, x2, w2, X)[0] == pytest.approx(l0, rel=1e-12)
+    assert 0.0 <= l0 < 1.0
+
+
 def test_loss_vanishes_for_the_exact_inverse_on_a_block_diagonal_system():
     s = ring_system(seed=4)
     off = s.row != s.col

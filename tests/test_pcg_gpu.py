@@ -58,9 +58,23 @@ def test_truncated_pcg_keeps_inexact_steps():
     params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100)
     direct, _ = run(prob, params)
     pcg, st = run(prob, params, use_pcg=1, pcg_max_iters=10, pcg_rel_tol=1e-8)
-    assert st["fallbacks"] == 0 and st["max_iterations"] <= 10
+    # every solve stops at 10 CG iterations and its iterate is used (no LDL^T fallback)
+    assert st["fallbacks"] == 0 and st["max_iterations"] == 10
+    assert st["iterations"] == 10 * st["solves"]
     best = lambda r: min(it["new_error"] for it in r["iterations"])
-    assert best(pcg) <= best(direct) * (1 + 1e-6)
+    assert best(pcg) < 0.02 * pcg["iterations"][0]["new_error"]  # LM still converges
+    assert best(pcg) <= best(direct) * (1 + 1e-2)
+
+
+def test_switch_to_exact_solves_reaches_direct_cost():
+    prob = capi.gen_ba_snavely(40, 3000, 13)
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100)
+    direct, _ = run(prob, params)
+    fast, st = run(prob, params, use_pcg=1, pcg_max_iters=20, pcg_rel_tol=1e-2,
+                   pcg_exact_below_reduction=1e-3)
+    best = lambda r: min(it["new_error"] for it in r["iterations"])
+    assert st["solves"] < len(fast["iterations"]) - 1  # the late iterations ran the LDL^T
+    assert best(fast) <= best(direct) * (1 + 1e-6)
 
 
 def test_pcg_rejected_on_cholesky_path():
