@@ -33,9 +33,19 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 METRIC = "LM iters/s and time to FP64-oracle cost (rel 1e-6), 1k-cam BA; fast vs plain"
 ORACLE_FIXTURE = REPO / "tests" / "golden" / "ba1k_oracle.json"
-# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, each scaled by the same
-# counters on a 512 MiB copy kernel (profiles/rcs_assemble_1k_ba_traffic.txt); None until measured
-TRAFFIC = {}
+TRAFFIC_PROFILE = REPO / "profiles" / "ba1k_plain_traffic.txt"
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed PMC summary (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counters on a 512 MiB copy
+    kernel); None when the summary is absent."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith(kernel_prefix):
+            return float(line.split()[-1]) * 1e6
+    return None
 
 
 def lm_params(iterations, early_exit=1e-6):
@@ -150,7 +160,8 @@ def roofline(r):
     fac_tf = info["flops"] / (fac_ms * 1e-3) / 1e12
     return dict(kernel="rcs_assemble_kernel<9>", bound="mfma", achieved=achieved,
                 peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                traffic=TRAFFIC.get("rcs_assemble"), ms_per_launch=ms, algorithmic_flops=flops,
+                traffic=measured_traffic("void rcs_assemble_kernel<9>"), ms_per_launch=ms,
+                algorithmic_flops=flops,
                 algorithmic_bytes=bytes_,
                 hbm_gbs=bytes_ / (ms * 1e-3) / 1e9,
                 ldlt_factor_phase={"ms": fac_ms, "flops": info["flops"], "tflops": fac_tf,

@@ -119,7 +119,8 @@ __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
                                                        double* __restrict__ gB) {
   using F = FT<FTC>;
   constexpr int NH = ND * (ND + 1) / 2;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // neighbouring cameras share landmarks: keep consecutive cameras on one XCD
+  const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wave >= n_nodes) return;
   double h[NH] = {}, g[ND] = {};
@@ -328,7 +329,9 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   static_assert(ND <= 16, "camera node wider than the MFMA tile");
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int NE = ND * ND;
-  const int blk = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  // blocks of one camera row share Z: consecutive logical workgroups stay on one XCD's L2
+  const int wg = xcd_tile(blockIdx.x, (n_blocks + 3) / 4);
+  const int blk = __builtin_amdgcn_readfirstlane(wg * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (blk >= n_blocks) return;
   const int a = blk_a[blk], b = blk_b[blk];
@@ -386,7 +389,7 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           const double* __restrict__ hp,
                                                           const double* __restrict__ gB,
                                                           double* __restrict__ v) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wave >= n_nodes) return;
   double acc[ND] = {};
@@ -963,7 +966,7 @@ void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
                            W_[b].get(), V_[b].get(), gp_[b].get());
       }
       PhaseScope ps(timer_, SYMHIP_PHASE_LIN_CAMS, s);
-      hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
+      hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
                          dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(), ov, values,
                          Bdiag_[b].get(), gB_[b].get());
     }
@@ -1015,7 +1018,7 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   auto assemble = [&](bool full) {
     if (!full) fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
     PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);  // exactly the one assembly kernel
-    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(grid_for((int64_t)n_blocks_ * 64, 256)),
+    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(xcd_grid((n_blocks_ + 3) / 4)),
                        dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
                        pairs_.get(), Z_.get(), Bdiag_[b].get(), Bfac_[b].get(), dnode_.get(),
                        panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels(),
@@ -1033,7 +1036,7 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   assemble(pcg_now);
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
-    hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(grid_for((int64_t)n_nodes_ * 64, 256)),
+    hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
                        dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
                        obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get());
   }

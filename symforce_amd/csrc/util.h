@@ -23,4 +23,15 @@ constexpr int kReduceBlocks = 1024;
 
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
+// XCD-aware workgroup order. The dispatcher hands workgroup i to XCD i % 8, each XCD with its own
+// L2; kernels whose neighbouring tiles share inputs launch xcd_grid(n) workgroups and process
+// logical tile xcd_tile(blockIdx.x, n): XCD k then owns the contiguous logical range
+// [k * per, (k + 1) * per). Tiles past n (padding) return at once.
+constexpr int kXcds = 8;
+inline int xcd_grid(int n_wg) { return ((n_wg + kXcds - 1) / kXcds) * kXcds; }
+__device__ __forceinline__ int xcd_tile(int wg, int n_wg) {
+  const int per = (n_wg + kXcds - 1) / kXcds;
+  return (wg % kXcds) * per + wg / kXcds;
+}
+
 }  // namespace symhip
