@@ -145,10 +145,13 @@ def speedup(plain, fast):
 def roofline(r):
     """The largest single kernel launch of an LM iteration is the reduced-camera-system assembly
     (one launch; its phase timer brackets exactly that kernel with HIP events on the optimizer's
-    stream). It runs one FP64 16x16x4 MFMA per observation pair: the useful work is
+    stream). It reduces every off-diagonal S block over its observation pairs on the FP64
+    matrix cores (K = 3 per pair, packed densely: 8 pairs per 6 MFMAs): the useful work is
     81 entries x 3 FMAs x 2 = 486 flops per pair, and the unique HBM bytes are the pair list, Z
-    once and the S blocks out. The supernodal LDL^T phase (74 level launches) is reported beside
-    it as a phase: flops from the symbolic analysis over the phase time."""
+    once and the S blocks out (the diagonal blocks' own-observation sums are reduced per camera
+    in the reduced-rhs kernel). The supernodal LDL^T phase (one diagonal+panel launch and one
+    update launch per level) is reported beside it as a phase: flops from the symbolic analysis
+    over the phase time."""
     info = r["info"]
     ph = r["phases"]
     ms = ph["rcs_assemble"][0] / max(1, ph["rcs_assemble"][1])

@@ -46,13 +46,14 @@ class SupernodalLDLT {
   };
   std::vector<SolveGraph> solve_graphs_;  // one per (b, x) pair seen (a few state blocks)
   CholSymbolic sym_;
-  DBuf<double> panel_, l11_, D_, y_;
+  DBuf<double> panel_, l11_, D_, y_, yout_, part_;
+  DBuf<int32_t> counter_;  // per-supernode arrival counters of the backward pass (self-resetting)
   DBuf<int64_t> asm_map_, scat_col_;
   DBuf<int32_t> rows_, perm_, upd_rel_, scat_rel_;
   DBuf<uint8_t> desc_;  // per-supernode descriptors (chol_numeric.hip)
   // work lists (flattened over levels)
-  std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_;
-  DBuf<int32_t> lvl_snodes_, trsm_items_, upd_items_;
+  std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_, lvl_solve_ptr_;
+  DBuf<int32_t> lvl_snodes_, trsm_items_, upd_items_, solve_items_;
 };
 
 }  // namespace symhip

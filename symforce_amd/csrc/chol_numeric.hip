@@ -34,14 +34,17 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
     panel[map[q]] = vals[q];
 }
 
-// Right-looking blocked LDL^T of the 64 x 64 (padded) diagonal block held as 4 x 4 register
-// patches: thread (ty, tx) owns rows ty + 16x, cols tx + 16y. Step m eliminates columns
+// Right-looking blocked LDL^T of a 64-column panel held as 4 x 4 register patches: rows
+// 0..63 are the diagonal block (padded with identity), rows 64..16 RX - 1 are below rows that are
+// eliminated in the same sweep (their final values are L21 = A21 L11^-T D^-1). Thread (ty, tx)
+// owns rows ty + 16x (x < RX) and cols tx + 16y (y < 4). Step m eliminates columns
 // k0 = 4m .. k0 + 3 (register column kb = k0 / 16, owner threads tx = k0 % 16 + u): the owners
 // publish the four columns through LDS (one barrier per step, double-buffered), every thread
 // factors the 4 x 4 diagonal block redundantly, forms w = c L4^-T for its rows and columns
 // (c = the published entries, L(i, k0 + u) = w_u / d_u) and applies
 // A(i, j) -= sum_u w_iu w_ju / d_u; the owners then store L (and D on the diagonal).
-__device__ __forceinline__ void ldlt4_block(const double (*cb)[kMaxCols], int k0, double (&d)[4],
+template <int W>
+__device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, double (&d)[4],
                                             double (&dinv)[4], double (&L4)[4][4]) {
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -60,8 +63,9 @@ __device__ __forceinline__ void ldlt4_block(const double (*cb)[kMaxCols], int k0
   }
 }
 
-__device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*cb)[4][kMaxCols],
-                                                 int ty, int tx) {
+template <int RX>
+__device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
+                                                     double (*cb)[4][16 * RX], int ty, int tx) {
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
 #pragma unroll 1
@@ -69,14 +73,14 @@ __device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*cb)
       const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
       if (uo >= 0 && uo < 4) {
 #pragma unroll
-        for (int x = 0; x < 4; ++x) cb[buf][uo][ty + 16 * x] = a[x][kb];
+        for (int x = 0; x < RX; ++x) cb[buf][uo][ty + 16 * x] = a[x][kb];
       }
       __syncthreads();
       double d[4], dinv[4], L4[4][4];
-      ldlt4_block(cb[buf], k0, d, dinv, L4);
-      double wr[4][4], wc[4][4];
+      ldlt4_block<16 * RX>(cb[buf], k0, d, dinv, L4);
+      double wr[RX][4], wc[4][4];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
+      for (int x = 0; x < RX; ++x) {
         const int i = ty + 16 * x;
         const bool live = i >= k0 + 4;
 #pragma unroll
@@ -101,18 +105,18 @@ __device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*cb)
       }
       // wr <- L(i, k0 + u) = w_iu / d_u
 #pragma unroll
-      for (int x = 0; x < 4; ++x)
+      for (int x = 0; x < RX; ++x)
 #pragma unroll
         for (int u = 0; u < 4; ++u) wr[x][u] *= dinv[u];
 #pragma unroll
-      for (int x = 0; x < 4; ++x)
+      for (int x = 0; x < RX; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y)
 #pragma unroll
           for (int u = 0; u < 4; ++u) a[x][y] -= wr[x][u] * wc[y][u];
       if (uo >= 0 && uo < 4) {
 #pragma unroll
-        for (int x = 0; x < 4; ++x) {
+        for (int x = 0; x < RX; ++x) {
           const int i = ty + 16 * x;
           double v = a[x][kb];
 #pragma unroll
@@ -131,24 +135,23 @@ __device__ __forceinline__ void ldlt64_registers(double (&a)[4][4], double (*cb)
   }
 }
 
-// Fused: each block factors its supernode's diagonal block (redundantly across the row chunks of
-// that supernode; block chunk 0 writes L11 to l11 and D) and then solves L21 = A21 L11^-T D^-1
-// for its own 64-row chunk of the panel, in place.
+// Fused diagonal factorization and panel solve: block (s, chunk) eliminates the 64 x 64 diagonal
+// block of supernode s together with the chunk's 64 below rows in one right-looking sweep (the
+// diagonal block is factored redundantly by every chunk of s; chunk 0 writes L11 to l11 and D),
+// leaving L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
 __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restrict__ items,
                                                         const SnDesc* __restrict__ desc,
                                                         double* __restrict__ panel,
                                                         double* __restrict__ l11,
                                                         double* __restrict__ D) {
-  __shared__ double cb[2][4][kMaxCols];
-  __shared__ double L[kMaxCols][kMaxCols + 1];  // L[j][k] = L11(j, k) for j > k, else 0
-  __shared__ double Dinv[kMaxCols];
+  constexpr int RX = 8;  // 4 row groups of the diagonal block + 4 of the chunk
+  __shared__ double cb[2][4][16 * RX];
   const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   double* P = panel + sd.panel_off;
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-  double a[4][4], w[4][4];
-  // diagonal block (padded with identity) and this chunk of the below rows, loaded together
+  double a[RX][4];
   // Unconditional loads from clamped (valid) addresses, then selects: a guarded load would be
   // compiled as a branch and wait per element.
 #pragma unroll
@@ -161,17 +164,9 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
       const int r = chunk * 64 + i;
       const double vw = P[(int64_t)jc * ld + (mb > 0 ? n + min(r, mb - 1) : ic)];
       a[x][y] = (i < n && j < n && i >= j) ? va : (i == j ? 1.0 : 0.0);
-      w[x][y] = (r < mb && j < n) ? vw : 0.0;
+      a[x + 4][y] = (r < mb && j < n) ? vw : 0.0;
     }
-  ldlt64_registers(a, cb, ty, tx);
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int i = ty + 16 * x, j = tx + 16 * y;
-      L[i][j] = i > j ? a[x][y] : 0.0;
-      if (i == j) Dinv[i] = 1.0 / a[x][y];
-    }
+  ldlt_panel_registers<RX>(a, cb, ty, tx);
   if (chunk == 0) {
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -182,58 +177,12 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
         if (i == j && i < n) D[sd.col_begin + i] = a[x][y];
       }
   }
-  __syncthreads();
-  // W L11^T = A21, four columns per step: the owners publish W(:, k0..k0+3) (all earlier
-  // blocks applied), every thread finishes them for its rows with the unit 4 x 4 block of L11 and
-  // applies them to its later columns.
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll 1
-    for (int mm = 0; mm < 4; ++mm) {
-      const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
-      if (uo >= 0 && uo < 4) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x) cb[buf][uo][ty + 16 * x] = w[x][kb];
-      }
-      __syncthreads();
-      double wf[4][4], lj[4][4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double c = cb[buf][u][ty + 16 * x];
-#pragma unroll
-          for (int v = 0; v < u; ++v) c -= wf[x][v] * L[k0 + u][k0 + v];
-          wf[x][u] = c;
-        }
-#pragma unroll
-      for (int y = 0; y < 4; ++y)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) lj[y][u] = L[tx + 16 * y][k0 + u];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-          for (int u = 0; u < 4; ++u) w[x][y] -= wf[x][u] * lj[y][u];
-      if (uo >= 0 && uo < 4) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-          double v = wf[x][0];
-#pragma unroll
-          for (int u = 1; u < 4; ++u)
-            if (u == uo) v = wf[x][u];
-          w[x][kb] = v;
-        }
-      }
-    }
-  }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
       const int r = chunk * 64 + ty + 16 * x, j = tx + 16 * y;
-      if (r < mb && j < n) P[(int64_t)j * ld + n + r] = w[x][y] * Dinv[j];
+      if (r < mb && j < n) P[(int64_t)j * ld + n + r] = a[x + 4][y];
     }
 }
 
@@ -305,10 +254,15 @@ __global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__
   }
 }
 
+// y = P b; also clears the backward pass's partial sums.
 __global__ void permute_kernel(int32_t n, const int32_t* __restrict__ perm,
-                               const double* __restrict__ b, double* __restrict__ y) {
+                               const double* __restrict__ b, double* __restrict__ y,
+                               double* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[perm[i]] = b[i];
+  if (i < n) {
+    y[perm[i]] = b[i];
+    part[i] = 0.0;
+  }
 }
 __global__ void unpermute_kernel(int32_t n, const int32_t* __restrict__ perm,
                                  const double* __restrict__ y, double* __restrict__ x) {
@@ -332,113 +286,130 @@ __device__ __forceinline__ void stage_l11(const double* Lc, int n, double (*Ls)[
   }
 }
 
-// Forward: y_own = L11^-1 y_own (one wavefront, lane i holds y_i, L11 from LDS);
-// y_below -= L21 y_own with atomics (siblings at the same level share ancestors). A thread takes
-// one below-row and all 64 (padded) columns: its 64 panel loads are independent, and the first
-// row chunk's loads are issued before the triangular solve so they overlap it. Columns past n
-// are clamped to column n - 1 and multiplied by ys = 0.
-__global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ list,
+// Value of v in lane k (k wave-uniform), through two scalar lane reads: a short dependency for
+// the triangular recurrences (an LDS-based shuffle costs a full LDS round trip per step).
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int2 u = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(u.x, k),
+                                              __builtin_amdgcn_readlane(u.y, k)));
+}
+
+constexpr int kSolveRows = 256;  // below-rows per solve block (one per thread in the forward pass)
+
+// Forward, block (s, chunk): y_own = L11^-1 y_own (one wavefront, lane r holds y_r, L11 from LDS;
+// every chunk block of s solves redundantly, chunk 0 writes the result to yout) and, for the
+// chunk's below-rows i, y(row i) -= L21(i, :) y_own with atomics into yacc (siblings at the same
+// level share ancestors). A thread takes one below-row and all 64 (padded) columns; its 64 panel
+// loads are issued before the triangular solve so they overlap it. Columns past n are clamped to
+// column n - 1 and multiplied by ys = 0.
+__global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ items,
                                                   const SnDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
                                                   const double* __restrict__ l11,
-                                                  double* __restrict__ y) {
+                                                  double* __restrict__ yacc,
+                                                  double* __restrict__ yout) {
   __shared__ double Ls[kMaxCols][kMaxCols + 1];
   __shared__ double ys[kMaxCols];
-  const int s = list[blockIdx.x];
+  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
-  const double* P = panel + sd.panel_off + n;  // below rows of column 0
-  double* yo = y + sd.col_begin;
+  const double* P = panel + sd.panel_off;
   const int32_t* rw = rows + sd.rows_off + n;
+  const int i = chunk * kSolveRows + threadIdx.x;
   double lrow[kMaxCols];
   {
-    const int i = min((int)threadIdx.x, max(mb - 1, 0));
+    // clamped into the panel (the diagonal block when there are no below-rows)
+    const int64_t r = mb > 0 ? n + min(i, mb - 1) : 0;
 #pragma unroll
-    for (int k = 0; k < kMaxCols; ++k) lrow[k] = mb > 0 ? P[(int64_t)min(k, n - 1) * ld + i] : 0.0;
+    for (int k = 0; k < kMaxCols; ++k) lrow[k] = P[(int64_t)min(k, n - 1) * ld + r];
   }
   stage_l11(l11 + sd.l11_off, n, Ls);
   double yi = 0;
-  if (threadIdx.x < kMaxCols && threadIdx.x < n) yi = yo[threadIdx.x];
+  if (threadIdx.x < n) yi = yacc[sd.col_begin + threadIdx.x];
   __syncthreads();
   if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
+    const int r = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kMaxCols; ++k) {
-      const double yk = __shfl(yi, k, 64);
-      yi -= Ls[i][k] * yk;
-    }
-    if (i < n) yo[i] = yi;
-    ys[i] = i < n ? yi : 0.0;
+    for (int k = 0; k < kMaxCols; ++k) yi -= Ls[r][k] * lane_value(yi, k);
+    if (chunk == 0 && r < n) yout[sd.col_begin + r] = yi;
+    ys[r] = r < n ? yi : 0.0;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < mb; i += blockDim.x) {
-    if (i >= (int)blockDim.x) {
-#pragma unroll
-      for (int k = 0; k < kMaxCols; ++k) lrow[k] = P[(int64_t)min(k, n - 1) * ld + i];
-    }
+  if (i < mb) {
     double acc = 0;
 #pragma unroll
     for (int k = 0; k < kMaxCols; ++k) acc += lrow[k] * ys[k];
-    unsafeAtomicAdd(&y[rw[i]], -acc);
+    unsafeAtomicAdd(&yacc[rw[i]], -acc);
   }
 }
 
-// Backward: y_own -= L21^T y_below, then y_own = L11^-T y_own by one wavefront from LDS. Wave w
-// takes columns w, w + 4, ..., w + 60 at once (16 accumulators per lane, lanes over the rows), so
-// each row step issues 16 independent coalesced column loads; the 16 sums are reduced across the
-// wave at the end. Columns past n are clamped and their sums dropped.
-__global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ list,
+// Backward, block (s, chunk): part(own) += L21(chunk rows, :)^T y(chunk rows) (wave w takes
+// columns w, w + 4, ..., w + 60: 16 accumulators per lane, lanes over the rows, reduced across the
+// wave), added atomically into part; the last chunk block of s to finish (a per-supernode arrival
+// counter, reset by that block for the next solve) then forms y_own = L11^-T (y_own - part) with
+// one wavefront. part must be zero on entry (the permutation kernel clears it).
+__global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ items,
                                                   const SnDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
                                                   const double* __restrict__ l11,
-                                                  double* __restrict__ y) {
+                                                  double* __restrict__ yout,
+                                                  double* __restrict__ part,
+                                                  int32_t* __restrict__ counter) {
   __shared__ double Ls[kMaxCols][kMaxCols + 1];
-  __shared__ double ys[kMaxCols];
-  __shared__ double yb[1024];
-  const int s = list[blockIdx.x];
+  __shared__ int last;
+  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
+  const int nchunks = max(1, (mb + kSolveRows - 1) / kSolveRows);
   const double* P = panel + sd.panel_off + n;
-  double* yo = y + sd.col_begin;
   const int32_t* rw = rows + sd.rows_off + n;
-  stage_l11(l11 + sd.l11_off, n, Ls);
-  const bool cached = mb <= 1024;
-  if (cached) {
-#pragma unroll 4
-    for (int i = threadIdx.x; i < mb; i += blockDim.x) yb[i] = y[rw[i]];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (mb > 0) {
+    constexpr int kPer = kMaxCols / 4;
+    double acc[kPer] = {};
+    int64_t coff[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) coff[c] = (int64_t)min(wave + 4 * c, n - 1) * ld;
+#pragma unroll
+    for (int t = 0; t < kSolveRows / 64; ++t) {
+      const int i = chunk * kSolveRows + lane + 64 * t;
+      const int ic = min(i, mb - 1);
+      const double yl = yout[rw[ic]];
+      const double yv = i < mb ? yl : 0.0;
+#pragma unroll
+      for (int c = 0; c < kPer; ++c) acc[c] += P[coff[c] + ic] * yv;
+    }
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      double v = acc[c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      const int k = wave + 4 * c;
+      if (lane == 0 && k < n) unsafeAtomicAdd(&part[sd.col_begin + k], v);
+    }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int kPer = kMaxCols / 4;
-  double acc[kPer] = {};
-  int64_t coff[kPer];
-#pragma unroll
-  for (int c = 0; c < kPer; ++c) coff[c] = (int64_t)min(wave + 4 * c, n - 1) * ld;
-  for (int i = lane; i < mb; i += 64) {
-    const double yv = cached ? yb[i] : y[rw[i]];
-#pragma unroll
-    for (int c = 0; c < kPer; ++c) acc[c] += P[coff[c] + i] * yv;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&counter[s], 1) == nchunks - 1;
   }
-#pragma unroll
-  for (int c = 0; c < kPer; ++c) {
-    double v = acc[c];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int k = wave + 4 * c;
-    if (lane == 0 && k < n) ys[k] = yo[k] - v;
-  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  stage_l11(l11 + sd.l11_off, n, Ls);
   __syncthreads();
   if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
-    double yi = i < n ? ys[i] : 0.0;
+    const int r = threadIdx.x;
+    double yi = 0;
+    if (r < n)
+      yi = yout[sd.col_begin + r] - __hip_atomic_load(&part[sd.col_begin + r], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-    for (int k = kMaxCols - 1; k >= 0; --k) {
-      const double yk = __shfl(yi, k, 64);
-      yi -= Ls[k][i] * yk;
-    }
-    if (i < n) yo[i] = yi;
+    for (int k = kMaxCols - 1; k >= 0; --k) yi -= Ls[k][r] * lane_value(yi, k);
+    if (r < n) yout[sd.col_begin + r] = yi;
+    if (r == 0) counter[s] = 0;
   }
 }
 
@@ -453,6 +424,10 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   panel_.resize(std::max<int64_t>(1, sym.panel_size()));
   D_.resize(std::max(1, sym.n));
   y_.resize(std::max(1, sym.n));
+  yout_.resize(std::max(1, sym.n));
+  part_.resize(std::max(1, sym.n));
+  counter_.resize(std::max(1, sym.n_snodes));
+  fill_zero_i32(counter_.get(), counter_.size(), s);
   rows_.upload(sym.sn_rows, s);
   perm_.upload(sym.perm, s);
   upd_rel_.upload(sym.upd_rel, s);
@@ -492,8 +467,9 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   scat_col_.upload(scol, s);
   scat_rel_.upload(srel, s);
   // Work lists per level
-  std::vector<int32_t> snl, dt, upd;
+  std::vector<int32_t> snl, dt, upd, sol;
   lvl_snode_ptr_.assign(1, 0);
+  lvl_solve_ptr_.assign(1, 0);
   lvl_trsm_ptr_.assign(1, 0);
   lvl_upd_ptr_.assign(1, 0);
   for (int l = 0; l < sym.n_levels; ++l) {
@@ -507,6 +483,11 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
         dt.push_back(sn);
         dt.push_back(c);
       }
+      const int schunks = std::max(1, (mb + kSolveRows - 1) / kSolveRows);
+      for (int c = 0; c < schunks; ++c) {
+        sol.push_back(sn);
+        sol.push_back(c);
+      }
       const int T = (mb + kTile - 1) / kTile;
       for (int ti = 0; ti < T; ++ti)
         for (int tj = 0; tj <= ti; ++tj) {
@@ -516,10 +497,12 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
         }
     }
     lvl_snode_ptr_.push_back((int32_t)snl.size());
+    lvl_solve_ptr_.push_back((int32_t)sol.size() / 2);
     lvl_trsm_ptr_.push_back((int32_t)dt.size() / 2);
     lvl_upd_ptr_.push_back((int32_t)upd.size() / 3);
   }
   lvl_snodes_.upload(snl, s);
+  solve_items_.upload(sol, s);
   trsm_items_.upload(dt, s);
   upd_items_.upload(upd, s);
   HIP_OK(hipStreamSynchronize(s));
@@ -597,22 +580,23 @@ void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
   const int n = sym_.n;
   const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(), b,
-                     y_.get());
+                     y_.get(), part_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
-    const int ns = lvl_snode_ptr_[l + 1] - lvl_snode_ptr_[l];
-    hipLaunchKernelGGL(fwd_kernel, dim3(ns), dim3(256), 0, s,
-                       lvl_snodes_.get() + lvl_snode_ptr_[l], desc, rows_.get(), panel_.get(),
-                       l11_.get(), y_.get());
+    const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
+    hipLaunchKernelGGL(fwd_kernel, dim3(nb), dim3(256), 0, s,
+                       solve_items_.get() + 2 * lvl_solve_ptr_[l], desc, rows_.get(),
+                       panel_.get(), l11_.get(), y_.get(), yout_.get());
   }
-  hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(), y_.get());
+  hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(),
+                     yout_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {
-    const int ns = lvl_snode_ptr_[l + 1] - lvl_snode_ptr_[l];
-    hipLaunchKernelGGL(bwd_kernel, dim3(ns), dim3(256), 0, s,
-                       lvl_snodes_.get() + lvl_snode_ptr_[l], desc, rows_.get(), panel_.get(),
-                       l11_.get(), y_.get());
+    const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
+    hipLaunchKernelGGL(bwd_kernel, dim3(nb), dim3(256), 0, s,
+                       solve_items_.get() + 2 * lvl_solve_ptr_[l], desc, rows_.get(),
+                       panel_.get(), l11_.get(), yout_.get(), part_.get(), counter_.get());
   }
   hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(),
-                     y_.get(), x);
+                     yout_.get(), x);
   HIP_OK(hipGetLastError());
 }
 

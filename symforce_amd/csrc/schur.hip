@@ -308,19 +308,25 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
   z[2] = z2;
 }
 
-// One wavefront per S block (a, b): S = [Bdiag + damping if a == b] + Bfac - sum_pairs Z_x Z_y^T.
-// The pair sum runs on the FP64 matrix cores, one v_mfma_f64_16x16x4f64 per observation pair
-// (x, y): lane l feeds A(i = l % 16, k = l / 16) = Z_x(i, k) and B(k, j = l % 16) = Z_y(j, k) for
-// k < 3 (k = 3 and rows i >= ND are zero padding) and holds D(l / 16 + 4 r, l % 16) in
-// accumulator register r (the FP64 16x16x4 output interleaves rows across the four lane groups). The pair indices are wave-uniform (scalar loads); 8 pairs' Z loads are
-// issued before their MFMAs. Output: into the LDL^T panels at (base + col * ld + row) or
+// One wavefront per S block (a, b): S = [Bdiag + damping - ZZ_a if a == b] + Bfac
+// - sum_pairs Z_x Z_y^T. The pair sum is the product X Y^T with X = [Z_x1 Z_x2 ...] (ND x 3P)
+// and Y = [Z_y1 Z_y2 ...], so K = 3P is packed densely into FP64 matrix-core steps
+// (v_mfma_f64_16x16x4f64; eight pairs = 24 K entries = six MFMAs): at step h lane l feeds
+// A(i = l % 16, k = l / 16) = X(i, 4h + k) and B(k, j = l % 16) = Y(j, 4h + k) (rows i >= ND and
+// K entries past the block's pairs are zero) and holds D(l / 16 + 4 r, l % 16) in accumulator
+// register r (the FP64 16x16x4 output interleaves rows across the four lane groups). The eight
+// pairs' observation indices are wave-uniform scalar loads (the pair list is padded so a batch
+// reaching past the last block stays inside it). Diagonal blocks carry no pairs: their
+// sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take blocks from a host-built
+// item list (-1 = idle wave). Output: into the LDL^T panels at (base + col * ld + row) or
 // transposed, or (sfull != null) into full symmetric block storage, block row-major, at
 // f_lo[blk] and its mirror f_up[blk] (transposed).
 template <int ND>
 __global__ void __launch_bounds__(256) rcs_assemble_kernel(
-    int32_t n_blocks, const int32_t* __restrict__ blk_a, const int32_t* __restrict__ blk_b,
-    const int32_t* __restrict__ pair_ptr, const int32_t* __restrict__ pairs,
-    const double* __restrict__ Z, const double* __restrict__ Bdiag,
+    int32_t n_items, const int32_t* __restrict__ items, const int32_t* __restrict__ blk_a,
+    const int32_t* __restrict__ blk_b, const int32_t* __restrict__ pair_ptr,
+    const int32_t* __restrict__ pairs, const double* __restrict__ Z,
+    const double* __restrict__ ZZ, const double* __restrict__ Bdiag,
     const double* __restrict__ Bfac, const double* __restrict__ dnode,
     const int64_t* __restrict__ panel_base, const int32_t* __restrict__ panel_ld,
     const int32_t* __restrict__ panel_trans, double* __restrict__ panel,
@@ -328,35 +334,57 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     const int32_t* __restrict__ f_up) {
   static_assert(ND <= 16, "camera node wider than the MFMA tile");
   typedef double d4 __attribute__((ext_vector_type(4)));
-  constexpr int NE = ND * ND;
-  // blocks of one camera row share Z: consecutive logical workgroups stay on one XCD's L2
-  const int wg = xcd_tile(blockIdx.x, (n_blocks + 3) / 4);
-  const int blk = __builtin_amdgcn_readfirstlane(wg * 4 + (threadIdx.x >> 6));
+  constexpr int NE = ND * ND, NZ = ND * 3;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= n_items) return;
+  const int blk = __builtin_amdgcn_readfirstlane(items[t]);
+  if (blk < 0) return;
   const int lane = threadIdx.x & 63;
-  if (blk >= n_blocks) return;
   const int a = blk_a[blk], b = blk_b[blk];
   const int row = lane & 15, kg = lane >> 4;
-  const bool live = row < ND && kg < 3;
-  const int zoff = live ? row * 3 + kg : 0;
-  d4 acc[4] = {};
+  const bool live = row < ND;
+  const int rowoff = (live ? row : 0) * 3;
+  d4 acc0 = {}, acc1 = {}, acc2 = {};
   const int q_begin = pair_ptr[blk], q_end = pair_ptr[blk + 1];
-  for (int q = q_begin; q < q_end; q += 8) {
-    double av[8], bv[8];
+  // K entry 4h + kg of a batch belongs to its pair p = (4h + kg) / 3 (column (4h + kg) % 3), which
+  // is pair plo = 4h / 3 or plo + 1: a select between two wave-uniform indices.
+  auto load_z = [&](int q, const int (&xs)[8], const int (&ys)[8], double (&av)[6],
+                    double (&bv)[6]) {
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      const bool okq = q + h < q_end;
-      const int qq = min(q + h, q_end - 1);  // in range; the tail's products are masked
-      const int xo = pairs[2 * qq], yo = pairs[2 * qq + 1];
-      const double za = Z[(int64_t)xo * (ND * 3) + zoff];
-      const double zb = Z[(int64_t)yo * (ND * 3) + zoff];
-      av[h] = (okq && live) ? za : 0.0;
-      bv[h] = (okq && live) ? zb : 0.0;
+    for (int h = 0; h < 6; ++h) {
+      const int K = 4 * h + kg;
+      const int p = K / 3, col = K - 3 * p;
+      const int plo = (4 * h) / 3;
+      const bool hi = p > plo;
+      const int x = hi ? xs[plo + 1] : xs[plo];
+      const int y = hi ? ys[plo + 1] : ys[plo];
+      const bool ok = live && q + p < q_end;
+      const double za = Z[(int64_t)x * NZ + rowoff + col];
+      const double zb = Z[(int64_t)y * NZ + rowoff + col];
+      av[h] = ok ? za : 0.0;
+      bv[h] = ok ? zb : 0.0;
     }
+  };
+  // Indices past the block's end come from the next block or the padding (valid observations),
+  // masked out by ok. Enough waves stay resident (about 90 registers each) to overlap one wave's
+  // loads with the others' MFMAs.
+  for (int q = q_begin; q < q_end; q += 8) {
+    int xs[8], ys[8];
 #pragma unroll
-    for (int h = 0; h < 8; ++h)
-      acc[h & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc[h & 3], 0, 0, 0);
+    for (int p = 0; p < 8; ++p) {
+      xs[p] = pairs[2 * (q + p)];
+      ys[p] = pairs[2 * (q + p) + 1];
+    }
+    double av[6], bv[6];
+    load_z(q, xs, ys, av, bv);
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[2], bv[2], acc2, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[3], bv[3], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[4], bv[4], acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[5], bv[5], acc2, 0, 0, 0);
   }
-  const d4 sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  const d4 sum = acc0 + acc1 + acc2;
   const int j = row;
   if (j >= ND) return;
 #pragma unroll
@@ -365,7 +393,9 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     if (i >= ND) continue;
     if (a == b && i < j) continue;
     double s = Bfac[(int64_t)blk * NE + i * ND + j] - sum[r];
-    if (a == b) s += Bdiag[(int64_t)a * NE + i * ND + j] + (i == j ? dnode[a * ND + i] : 0.0);
+    if (a == b)
+      s += Bdiag[(int64_t)a * NE + i * ND + j] - ZZ[(int64_t)a * NE + i * ND + j] +
+           (i == j ? dnode[a * ND + i] : 0.0);
     if (sfull) {
       sfull[(int64_t)f_lo[blk] * NE + i * ND + j] = s;
       sfull[(int64_t)f_up[blk] * NE + j * ND + i] = s;
@@ -379,7 +409,8 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   }
 }
 
-// Per camera node (wavefront): v_a = g_a - sum_{o in a} Z_o h_p(o).
+// Per camera node (wavefront): v_a = g_a - sum_{o in a} Z_o h_p(o) and the diagonal-block
+// elimination term ZZ_a = sum_{o in a} Z_o Z_o^T (full ND x ND, row-major).
 template <int ND>
 __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           const int32_t* __restrict__ cam_obs_ptr,
@@ -388,25 +419,50 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           const double* __restrict__ Z,
                                                           const double* __restrict__ hp,
                                                           const double* __restrict__ gB,
-                                                          double* __restrict__ v) {
+                                                          double* __restrict__ v,
+                                                          double* __restrict__ ZZ) {
+  constexpr int NH = ND * (ND + 1) / 2;
   const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wave >= n_nodes) return;
-  double acc[ND] = {};
+  double acc[ND] = {}, zz[NH] = {};
   for (int q = cam_obs_ptr[wave] + lane; q < cam_obs_ptr[wave + 1]; q += 64) {
     const int o = cam_obs[q];
     const int p = obs_point[o];
     const double h0 = hp[p * 3], h1 = hp[p * 3 + 1], h2 = hp[p * 3 + 2];
     const double* Zo = Z + (int64_t)o * ND * 3;
+    double z[ND * 3];
 #pragma unroll
-    for (int i = 0; i < ND; ++i) acc[i] += Zo[i * 3] * h0 + Zo[i * 3 + 1] * h1 + Zo[i * 3 + 2] * h2;
+    for (int e = 0; e < ND * 3; ++e) z[e] = Zo[e];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) acc[i] += z[i * 3] * h0 + z[i * 3 + 1] * h1 + z[i * 3 + 2] * h2;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int k = 0; k <= i; ++k)
+        zz[idx++] += z[i * 3] * z[k * 3] + z[i * 3 + 1] * z[k * 3 + 1] + z[i * 3 + 2] * z[k * 3 + 2];
   }
 #pragma unroll
   for (int i = 0; i < ND; ++i)
     for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_down(acc[i], off, 64);
-  if (lane == 0)
+#pragma unroll
+  for (int i = 0; i < NH; ++i)
+    for (int off = 32; off > 0; off >>= 1) zz[i] += __shfl_down(zz[i], off, 64);
+  if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) v[wave * ND + i] = gB[wave * ND + i] - acc[i];
+    double* zo = ZZ + (int64_t)wave * ND * ND;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int k = 0; k <= i; ++k) {
+        zo[i * ND + k] = zz[idx];
+        zo[k * ND + i] = zz[idx];
+        ++idx;
+      }
+  }
 }
 
 // Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
@@ -517,7 +573,7 @@ class SchurBackend final : public SystemBackend {
   // observation tables
   DBuf<int32_t> obs_arg_off_, obs_res_off_, obs_node_, obs_point_, pt_obs_ptr_, cam_obs_ptr_,
       cam_obs_;
-  DBuf<int32_t> node_tan_, pt_tan_, blk_a_, blk_b_, pair_ptr_, pairs_, diag_blk_;
+  DBuf<int32_t> node_tan_, pt_tan_, blk_a_, blk_b_, pair_ptr_, pairs_, diag_blk_, rcs_items_;
   DBuf<int64_t> panel_base_;
   DBuf<int32_t> panel_ld_, panel_trans_;
   // camera-only factors
@@ -529,7 +585,7 @@ class SchurBackend final : public SystemBackend {
   std::vector<BGroup> bgroups_;
   // per state block storage
   DBuf<double> residual_[3], W_[3], V_[3], gp_[3], Bdiag_[3], Bfac_[3], gB_[3], rhs_[3];
-  DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, v_, y_, partials_;
+  DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, ZZ_, v_, y_, partials_;
   SupernodalLDLT chol_;
   std::unique_ptr<CscLinearizer> csc_;  // reference-layout H for best-linearization output
   // Iterative form: PCG on the reduced camera system in full block storage, falling back to the
@@ -759,28 +815,55 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   };
   std::vector<int32_t> diag_blk(n_nodes_);
   for (int32_t a = 0; a < n_nodes_; ++a) diag_blk[a] = block_index(a, a);
-  // Observation pairs per block
+  // Observation pairs per off-diagonal block (a > b; the diagonal blocks' sum over the camera's
+  // own observations is reduced per camera instead). The list is padded with 8 pairs of
+  // observation 0 so the assembly kernel's batches of 8 stay inside it.
   std::vector<int32_t> pcount(n_blocks_ + 1, 0);
   for (int32_t p = 0; p < n_points_; ++p)
     for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
       for (int32_t y = pt_ptr[p]; y < pt_ptr[p + 1]; ++y) {
         const int32_t a = obs_node[x], b = obs_node[y];
-        if (a >= b) pcount[block_index(a, b) + 1] += 1;
+        if (a > b) pcount[block_index(a, b) + 1] += 1;
       }
   for (int32_t k = 0; k < n_blocks_; ++k) pcount[k + 1] += pcount[k];
   n_pairs_ = pcount[n_blocks_];
-  std::vector<int32_t> pairs(2 * (size_t)pcount[n_blocks_]);
+  std::vector<int32_t> pairs(2 * ((size_t)pcount[n_blocks_] + 8), 0);
   {
     std::vector<int32_t> fill(pcount.begin(), pcount.end() - 1);
     for (int32_t p = 0; p < n_points_; ++p)
       for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
         for (int32_t y = pt_ptr[p]; y < pt_ptr[p + 1]; ++y) {
           const int32_t a = obs_node[x], b = obs_node[y];
-          if (a < b) continue;
+          if (a <= b) continue;
           const int32_t q = fill[block_index(a, b)]++;
           pairs[2 * (size_t)q] = x;
           pairs[2 * (size_t)q + 1] = y;
         }
+  }
+  // Assembly work list. Workgroups are dispatched round-robin over the XCDs (XCD = workgroup
+  // index mod kXcds), each with its own L2: XCD x takes a contiguous range of camera rows holding
+  // about 1/kXcds of the pairs (blocks of one row share the row camera's Z), heaviest blocks
+  // first so the longest waves start early; -1 pads the XCDs to equal length.
+  std::vector<int32_t> rcs_items;
+  {
+    std::vector<int64_t> before(n_nodes_ + 1, 0);  // pairs (+1 per block) in rows < a
+    for (int32_t k = 0; k < n_blocks_; ++k) before[blk_a[k] + 1] += pcount[k + 1] - pcount[k] + 1;
+    for (int32_t a = 0; a < n_nodes_; ++a) before[a + 1] += before[a];
+    std::vector<std::vector<int32_t>> per(kXcds);
+    for (int32_t k = 0; k < n_blocks_; ++k)
+      per[std::min<int64_t>(kXcds - 1, before[blk_a[k]] * kXcds / before[n_nodes_])].push_back(k);
+    size_t len = 0;
+    for (auto& v : per) {
+      std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
+        return pcount[x + 1] - pcount[x] > pcount[y + 1] - pcount[y];
+      });
+      len = std::max(len, v.size());
+    }
+    len = (len + 3) / 4 * 4;
+    rcs_items.assign(kXcds * len, -1);
+    for (int x = 0; x < kXcds; ++x)
+      for (size_t i = 0; i < per[x].size(); ++i)
+        rcs_items[((i / 4) * kXcds + x) * 4 + i % 4] = per[x][i];
   }
   // RCS symbolic: nodes = cameras (uniform nd), lower CSC pattern of the dense blocks.
   const int32_t nB = n_nodes_ * nd_;
@@ -924,6 +1007,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   blk_b_.upload(blk_b, s);
   pair_ptr_.upload(pcount, s);
   pairs_.upload(pairs, s);
+  rcs_items_.upload(rcs_items, s);
   diag_blk_.upload(diag_blk, s);
   panel_base_.upload(pbase, s);
   panel_ld_.upload(pld, s);
@@ -946,6 +1030,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   Lp_.resize((size_t)n_points_ * 6);
   hp_.resize((size_t)n_points_ * 3);
   Z_.resize((size_t)n_obs_ * nd_ * 3);
+  ZZ_.resize((size_t)n_nodes_ * nde);
   v_.resize((size_t)n_nodes_ * nd_);
   y_.resize((size_t)n_nodes_ * nd_);
   partials_.resize(kReduceBlocks);
@@ -1018,9 +1103,11 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   auto assemble = [&](bool full) {
     if (!full) fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
     PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);  // exactly the one assembly kernel
-    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(xcd_grid((n_blocks_ + 3) / 4)),
-                       dim3(256), 0, s, n_blocks_, blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
-                       pairs_.get(), Z_.get(), Bdiag_[b].get(), Bfac_[b].get(), dnode_.get(),
+    const int32_t n_items = (int32_t)rcs_items_.size();
+    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(n_items / 4), dim3(256), 0, s, n_items,
+                       rcs_items_.get(), blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
+                       pairs_.get(), Z_.get(), ZZ_.get(), Bdiag_[b].get(), Bfac_[b].get(),
+                       dnode_.get(),
                        panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels(),
                        full ? pcg_.blocks() : nullptr, f_lo_.get(), f_up_.get());
   };
@@ -1033,13 +1120,13 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     chol_.solve(v_.get(), y_.get(), s);
   };
   const bool pcg_now = use_pcg_ && !exact_now_;
-  assemble(pcg_now);
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
     hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
                        dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
-                       obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get());
+                       obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get(), ZZ_.get());
   }
+  assemble(pcg_now);
   if (pcg_now && (int)logged_S_.size() < log_max_) {
     DBuf<double> Sc, vc;
     Sc.resize((size_t)pcg_.n_blocks() * ND * ND);
