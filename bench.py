@@ -81,11 +81,14 @@ def run_gpu(args):
     target = None
     if ORACLE_FIXTURE.exists():
         target = json.loads(ORACLE_FIXTURE.read_text())["final_cost"]
-    plain = fast = None
+    plain = fast = fast_alt = None
     if not args.no_converge:
         plain = time_to_cost(prob, opts, target)
         if not args.no_fast:
             fast = time_to_cost(prob, fast_options(args), target)
+            alt = "f64" if args.fast_precision == "f32" else "f32"
+            fast_alt = dict(precision=alt,
+                            **time_to_cost(prob, fast_options(args, precision=alt), target))
     # the fast form's iteration rate and phases over the same K steps
     fast_ms, fast_phases, fast_lin = None, {}, None
     if not args.no_fast:
@@ -101,15 +104,21 @@ def run_gpu(args):
         fast_lin = fopt.linear_solver_stats()
     return dict(prob=prob, opt=opt, info=info, t_gen=t_gen, t_setup=t_setup,
                 ms_per_step=ms_per_step, n_iter=n_iter, phases=phases, res=res, plain=plain,
-                fast=fast, target=target, fast_ms=fast_ms, fast_phases=fast_phases,
+                fast=fast, fast_alt=fast_alt, target=target, fast_ms=fast_ms,
+                fast_phases=fast_phases,
                 fast_lin=fast_lin)
 
 
-def fast_options(args, switch=True):
-    """The fast form: block-Jacobi PCG on the reduced camera system (FP64), falling back to the
-    supernodal LDL^T when CG does not reach the tolerance."""
+def fast_options(args, switch=True, precision=None):
+    """The fast form: block-Jacobi PCG on the reduced camera system (truncated at the tolerance or
+    the iteration cap), in the FP32 precision form by default (W, Z and S stored in FP32, FP64
+    sums, cost and decisions), switching to the exact supernodal LDL^T once an accepted step's
+    relative reduction falls below --pcg-exact-below."""
+    prec = args.fast_precision if precision is None else precision
     return capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
                                ordering=capi.ORDERING_NESTED_DISSECTION,
+                               precision=capi.PRECISION_F32_JAC if prec == "f32" else
+                               capi.PRECISION_F64,
                                use_pcg=1, pcg_max_iters=args.pcg_max_iters,
                                pcg_rel_tol=args.pcg_tol,
                                pcg_exact_below_reduction=args.pcg_exact_below if switch else 0.0)
@@ -161,9 +170,9 @@ def roofline(r):
     achieved = flops / (ms * 1e-3) / 1e12
     fac_ms = ph["factor"][0] / max(1, ph["factor"][1])
     fac_tf = info["flops"] / (fac_ms * 1e-3) / 1e12
-    return dict(kernel="rcs_assemble_kernel<9>", bound="mfma", achieved=achieved,
+    return dict(kernel="rcs_assemble_kernel<9, double, double>", bound="mfma", achieved=achieved,
                 peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                traffic=measured_traffic("void rcs_assemble_kernel<9>"), ms_per_launch=ms,
+                traffic=measured_traffic("void rcs_assemble_kernel<9, double, double>"), ms_per_launch=ms,
                 algorithmic_flops=flops,
                 algorithmic_bytes=bytes_,
                 hbm_gbs=bytes_ / (ms * 1e-3) / 1e9,
@@ -225,6 +234,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
+    ap.add_argument("--fast-precision", choices=("f32", "f64"), default="f32",
+                    help="storage precision of the fast form's W, Z and S")
     ap.add_argument("--pcg-tol", type=float, default=1e-2)
     ap.add_argument("--pcg-max-iters", type=int, default=30)
     ap.add_argument("--pcg-exact-below", type=float, default=3e-4,
@@ -259,7 +270,10 @@ def main():
                    "n_obs": int(r["prob"].info["n_obs"]), "solver": "schur+supernodal_ldlt", "ordering": "nested_dissection",
                    "lambda": "dynamic", "form": "plain_fp64"},
         "time_to_cost": {"target_cost": r["target"], "plain": r["plain"], "fast": r["fast"],
-                         "fast_form": {"solver": "schur+block_jacobi_pcg", "dtype": "f64",
+                         "fast_other_precision": r["fast_alt"],
+                         "fast_form": {"solver": "schur+block_jacobi_pcg",
+                                       "dtype": ("f32 W/Z/S storage, f64 sums/cost"
+                                                 if args.fast_precision == "f32" else "f64"),
                                        "pcg_rel_tol": args.pcg_tol,
                                        "pcg_max_iters": args.pcg_max_iters,
                                        "exact_ldlt_below_reduction": args.pcg_exact_below},

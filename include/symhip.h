@@ -96,7 +96,12 @@ enum symhip_precision {
 typedef struct symhip_solver_options {
   int32_t linear_solver; /* symhip_linear_solver */
   int32_t ordering;      /* symhip_ordering (fill-reducing ordering of the factorized matrix) */
-  int32_t precision;     /* symhip_precision */
+  int32_t precision;     /* symhip_precision. SYMHIP_PRECISION_F32_JAC on the optimizer: Schur
+                            path only; the Hessian-side products (W = J_cam^T J_point, V) come
+                            from FP32 factor bodies, W and Z = W L_p^-T are stored in FP32 and so
+                            is S for PCG; residual, gradient, matrix-core sums, LDL^T, cost and
+                            gain ratio stay FP64. Not combinable with the learned preconditioner
+                            or system logging (both need FP64 systems). */
   int32_t use_pcg;       /* Schur path: block-Jacobi preconditioned CG on the reduced camera
                             system. 0 off (direct LDL^T), 1 truncated (an iterate that missed
                             pcg_rel_tol within pcg_max_iters is used; LDL^T only when CG

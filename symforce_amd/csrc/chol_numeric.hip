@@ -42,7 +42,11 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
 // publish the four columns through LDS (one barrier per step, double-buffered), every thread
 // factors the 4 x 4 diagonal block redundantly, forms w = c L4^-T for its rows and columns
 // (c = the published entries, L(i, k0 + u) = w_u / d_u) and applies
-// A(i, j) -= sum_u w_iu w_ju / d_u; the owners then store L (and D on the diagonal).
+// A(i, j) -= sum_u w_iu w_ju / d_u. The finished columns go to Lout (LDS; D on the diagonal),
+// written by the tx == 0 thread of each row. Nothing is masked: entries above the diagonal and
+// columns already eliminated pick up products of other such entries, but they are never read
+// again (the final values are in Lout), and the entries still to be eliminated only ever combine
+// published lower-triangle values.
 template <int W>
 __device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, double (&d)[4],
                                             double (&dinv)[4], double (&L4)[4][4]) {
@@ -63,9 +67,12 @@ __device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, doubl
   }
 }
 
+constexpr int kLoutLd = kMaxCols + 2;  // Lout row stride (doubles): 16-byte rows, fewer conflicts
+
 template <int RX>
 __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
-                                                     double (*cb)[4][16 * RX], int ty, int tx) {
+                                                     double (*cb)[4][16 * RX],
+                                                     double (*Lout)[kLoutLd], int ty, int tx) {
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
 #pragma unroll 1
@@ -82,10 +89,9 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
 #pragma unroll
       for (int x = 0; x < RX; ++x) {
         const int i = ty + 16 * x;
-        const bool live = i >= k0 + 4;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          double c = live ? cb[buf][u][i] : 0.0;
+          double c = cb[buf][u][i];
 #pragma unroll
           for (int v = 0; v < u; ++v) c -= wr[x][v] * L4[u][v];
           wr[x][u] = c;
@@ -94,10 +100,9 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int j = tx + 16 * y;
-        const bool live = j >= k0 + 4;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          double c = live ? cb[buf][u][j] : 0.0;
+          double c = cb[buf][u][j];
 #pragma unroll
           for (int v = 0; v < u; ++v) c -= wc[y][v] * L4[u][v];
           wc[y][u] = c;
@@ -114,39 +119,49 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
         for (int y = 0; y < 4; ++y)
 #pragma unroll
           for (int u = 0; u < 4; ++u) a[x][y] -= wr[x][u] * wc[y][u];
-      if (uo >= 0 && uo < 4) {
+      if (tx == 0) {
+        // rows below the 4 x 4 block (row groups x > kb, and x == kb past it); the block itself
+        // (rows k0 .. k0 + 3, all in group kb) gets D and L4
 #pragma unroll
-        for (int x = 0; x < RX; ++x) {
+        for (int x = kb; x < RX; ++x) {
           const int i = ty + 16 * x;
-          double v = a[x][kb];
+          if (x == kb && i < k0 + 4) {
+            if (i >= k0) {
+              const int r = i - k0;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (u != uo) continue;
-            if (i >= k0 + 4) v = wr[x][u];
-            else if (i == k0 + u) v = d[u];
+              for (int u = 0; u < 4; ++u) {
+                double v = 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (r > u && i == k0 + r) v = L4[r][u];
+                for (int rr = 0; rr < 4; ++rr)
+                  if (rr == r) v = u < rr ? L4[rr][u] : (u == rr ? d[u] : 0.0);
+                Lout[i][k0 + u] = v;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Lout[i][k0 + u] = wr[x][u];
           }
-          a[x][kb] = v;
         }
       }
     }
   }
 }
 
-// Fused diagonal factorization and panel solve: block (s, chunk) eliminates the 64 x 64 diagonal
-// block of supernode s together with the chunk's 64 below rows in one right-looking sweep (the
-// diagonal block is factored redundantly by every chunk of s; chunk 0 writes L11 to l11 and D),
-// leaving L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
-__global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restrict__ items,
-                                                        const SnDesc* __restrict__ desc,
-                                                        double* __restrict__ panel,
-                                                        double* __restrict__ l11,
-                                                        double* __restrict__ D) {
-  constexpr int RX = 8;  // 4 row groups of the diagonal block + 4 of the chunk
-  __shared__ double cb[2][4][16 * RX];
-  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
+// LDS (doubles) of the two factorization kernels.
+constexpr int kDiagRX = 8;  // row groups of the diagonal task: 4 of the diagonal block + 4 below
+constexpr int kDiagSmem = 2 * 4 * 16 * kDiagRX + 16 * kDiagRX * kLoutLd;
+constexpr int kUpdSmem = 2 * kMaxCols * (kTile + 1);
+
+// Diagonal task (s, chunk), one 256-thread block: eliminates the 64 x 64 diagonal block of
+// supernode s together with the chunk's 64 below rows in one right-looking sweep (the diagonal
+// block is factored redundantly by every chunk of s; chunk 0 writes L11 to l11 and D), leaving
+// L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
+__device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __restrict__ desc,
+                                          double* __restrict__ panel, double* __restrict__ l11,
+                                          double* __restrict__ D, double* smem) {
+  constexpr int RX = kDiagRX;
+  auto cb = reinterpret_cast<double (*)[4][16 * RX]>(smem);
+  auto Lout = reinterpret_cast<double (*)[kLoutLd]>(smem + 2 * 4 * 16 * RX);
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   double* P = panel + sd.panel_off;
@@ -166,15 +181,17 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
       a[x][y] = (i < n && j < n && i >= j) ? va : (i == j ? 1.0 : 0.0);
       a[x + 4][y] = (r < mb && j < n) ? vw : 0.0;
     }
-  ldlt_panel_registers<RX>(a, cb, ty, tx);
+  ldlt_panel_registers<RX>(a, cb, Lout, ty, tx);
+  __syncthreads();
   if (chunk == 0) {
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int i = ty + 16 * x, j = tx + 16 * y;
-        if (i < n && j < n && i > j) l11[sd.l11_off + (int64_t)j * n + i] = a[x][y];
-        if (i == j && i < n) D[sd.col_begin + i] = a[x][y];
+        const double v = Lout[i][j];
+        if (i < n && j < n && i > j) l11[sd.l11_off + (int64_t)j * n + i] = v;
+        if (i == j && i < n) D[sd.col_begin + i] = v;
       }
   }
 #pragma unroll
@@ -182,24 +199,33 @@ __global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restric
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
       const int r = chunk * 64 + ty + 16 * x, j = tx + 16 * y;
-      if (r < mb && j < n) P[(int64_t)j * ld + n + r] = a[x + 4][y];
+      const double v = Lout[64 + ty + 16 * x][j];
+      if (r < mb && j < n) P[(int64_t)j * ld + n + r] = v;
     }
 }
 
-// U = L21 D L21^T tile (ti, tj), scattered as -U into ancestor panels. For below-row j of the
-// supernode, scat_col[j] is the panel offset of its column in the target supernode and
-// scat_rel[j] + i indexes upd_rel for the target row position of below-row i >= j.
-__global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__ items,
-                                                     const SnDesc* __restrict__ desc,
-                                                     double* __restrict__ panel,
-                                                     const double* __restrict__ D,
-                                                     const int64_t* __restrict__ scat_col,
-                                                     const int32_t* __restrict__ scat_rel,
-                                                     const int32_t* __restrict__ upd_rel) {
-  __shared__ double As[kMaxCols][kTile + 1];
-  __shared__ double Bs[kMaxCols][kTile + 1];
-  const int s = items[3 * blockIdx.x], ti = items[3 * blockIdx.x + 1],
-            tj = items[3 * blockIdx.x + 2];
+__global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restrict__ items,
+                                                        const SnDesc* __restrict__ desc,
+                                                        double* __restrict__ panel,
+                                                        double* __restrict__ l11,
+                                                        double* __restrict__ D) {
+  __shared__ double smem[kDiagSmem];
+  diag_body(items[2 * blockIdx.x], items[2 * blockIdx.x + 1], desc, panel, l11, D, smem);
+}
+
+// Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj), scattered as -U
+// into ancestor panels. For below-row j of the supernode, scat_col[j] is the panel offset of its
+// column in the target supernode and scat_rel[j] + i indexes upd_rel for the target row position
+// of below-row i >= j (i < mb).
+__device__ __forceinline__ void update_body(int s, int ti, int tj,
+                                            const SnDesc* __restrict__ desc,
+                                            double* __restrict__ panel,
+                                            const double* __restrict__ D,
+                                            const int64_t* __restrict__ scat_col,
+                                            const int32_t* __restrict__ scat_rel,
+                                            const int32_t* __restrict__ upd_rel, double* smem) {
+  auto As = reinterpret_cast<double (*)[kTile + 1]>(smem);
+  auto Bs = reinterpret_cast<double (*)[kTile + 1]>(smem + kMaxCols * (kTile + 1));
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const double* P = panel + sd.panel_off;
@@ -218,14 +244,20 @@ __global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__
     As[k][r] = (k < n && ri < mb) ? va : 0.0;
     Bs[k][r] = (k < n && rj < mb) ? vb : 0.0;
   }
-  // scatter metadata for this thread's columns, fetched while the tiles load
+  // scatter targets of this thread's 16 entries, fetched while the tiles load and before the
+  // product loop (the row positions depend on the metadata: two dependent loads, off the
+  // critical path here). Indices are clamped into the column's update segment (rows
+  // j .. mb - 1); clamped entries are masked at the scatter.
   int64_t cbase[4];
-  int32_t rbase[4];
+  int32_t rel[4][4];
 #pragma unroll
   for (int y = 0; y < 4; ++y) {
-    const int j = min(tj * kTile + tx + 16 * y, mb - 1);  // clamped; masked at the scatter
+    const int j = min(tj * kTile + tx + 16 * y, mb - 1);
     cbase[y] = scat_col[sd.rows_off + n + j];
-    rbase[y] = scat_rel[sd.rows_off + n + j];
+    const int32_t rbase = scat_rel[sd.rows_off + n + j];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+      rel[y][x] = upd_rel[rbase + min(max(ti * kTile + ty + 16 * x, j), mb - 1)];
   }
   __syncthreads();
   double acc[4][4] = {};
@@ -248,10 +280,21 @@ __global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
       const int i = ti * kTile + ty + 16 * x;
-      if (j < mb && i < mb && i >= j)
-        unsafeAtomicAdd(panel + cbase[y] + upd_rel[rbase[y] + i], -acc[x][y]);
+      if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[y] + rel[y][x], -acc[x][y]);
     }
   }
+}
+
+__global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__ items,
+                                                     const SnDesc* __restrict__ desc,
+                                                     double* __restrict__ panel,
+                                                     const double* __restrict__ D,
+                                                     const int64_t* __restrict__ scat_col,
+                                                     const int32_t* __restrict__ scat_rel,
+                                                     const int32_t* __restrict__ upd_rel) {
+  __shared__ double smem[kUpdSmem];
+  update_body(items[3 * blockIdx.x], items[3 * blockIdx.x + 1], items[3 * blockIdx.x + 2], desc,
+              panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
 // y = P b; also clears the backward pass's partial sums.
@@ -396,7 +439,7 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ it
     last = atomicAdd(&counter[s], 1) == nchunks - 1;
   }
   __syncthreads();
-  if (!last) return;
+  if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
   __threadfence();
   stage_l11(l11 + sd.l11_off, n, Ls);
   __syncthreads();

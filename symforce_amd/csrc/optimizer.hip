@@ -252,7 +252,9 @@ std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const Solv
 DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_optimizer_params& p,
                                  const SolverOptions& o, double epsilon)
     : p_(p), o_(o), epsilon_(epsilon) {
-  SYMHIP_CHECK(o.precision == 0, "symhip: only SYMHIP_PRECISION_F64 is implemented");
+  SYMHIP_CHECK(o.precision == SYMHIP_PRECISION_F64 ||
+                   (o.precision == SYMHIP_PRECISION_F32_JAC && o.linear_solver == kSolverSchur),
+               "symhip: SYMHIP_PRECISION_F32_JAC is available on the Schur path only");
   SYMHIP_CHECK(!o.use_pcg || o.linear_solver == kSolverSchur,
                "symhip: PCG is available on the Schur path only");
   build_host_problem(d, host_);

@@ -21,9 +21,12 @@ struct PcgResult {
 class BlockPcg {
  public:
   // row_ptr (n + 1) / col: block pattern of the full matrix; diag[a] = block index of (a, a).
+  // f32: the blocks are stored in FP32 (blocks32(); the sums stay FP64), else FP64 (blocks()).
   void setup(int nd, int32_t n_nodes, const std::vector<int32_t>& row_ptr,
-             const std::vector<int32_t>& col, const std::vector<int32_t>& diag, hipStream_t s);
+             const std::vector<int32_t>& col, const std::vector<int32_t>& diag, hipStream_t s,
+             bool f32 = false);
   double* blocks() const { return S_.get(); }
+  float* blocks32() const { return S32_.get(); }
   double* precond() const { return Minv_.get(); }
   int64_t n_blocks() const { return n_blk_; }
   // Minv[a] = inverse(S_aa) (+ correction[a] if given, device, n * nd * nd).
@@ -34,9 +37,8 @@ class BlockPcg {
   PcgResult solve(const double* b, double* x, int max_iters, double rel_tol, hipStream_t s,
                   int check_every = kChunk);
   static constexpr int kChunk = 16;  // even: a chunk ends on slot 1 and the next starts on 0
-  // y = S x (device vectors), for checks and step-error measurement.
-  void multiply(const double* x, double* y, hipStream_t s) { multiply_with(S_.get(), x, y, s); }
-  // y = S x for other blocks S of the same pattern.
+  // y = S x for FP64 blocks S of this pattern (device vectors), for checks and step-error
+  // measurement.
   void multiply_with(const double* S, const double* x, double* y, hipStream_t s);
 
  private:
@@ -45,7 +47,9 @@ class BlockPcg {
   int64_t n_blk_ = 0;
   int rows_per_block_ = 0, n_upd_blocks_ = 0;
   DBuf<int32_t> row_ptr_, col_, diag_;
+  bool f32_ = false;
   DBuf<double> S_, Minv_, r_, z_, p_[2], q_, partA_, partB_, scal_;
+  DBuf<float> S32_;
   double* host_scal_ = nullptr;  // pinned
   hipGraphExec_t chunk_ = nullptr;  // kChunk iterations, captured for (x, max_iters, tol)
   double* chunk_x_ = nullptr;

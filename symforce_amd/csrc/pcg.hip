@@ -31,18 +31,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Gauss-Jordan inverse of the SPD diagonal block (one wavefront per node), on [S_aa | I] in LDS.
 // A non-positive pivot (the block is not SPD) falls back to the inverse diagonal (Jacobi).
-template <int ND>
+template <int ND, typename TS>
 __global__ void __launch_bounds__(64) precond_kernel(int32_t n, const int32_t* __restrict__ diag,
-                                                     const double* __restrict__ S,
+                                                     const TS* __restrict__ S,
                                                      const double* __restrict__ corr,
                                                      double* __restrict__ Minv) {
   constexpr int W = 2 * ND, NA = ND * W, NE = ND * ND, PER = (NA + 63) / 64;
   __shared__ double A[ND][W + 1];
   const int a = blockIdx.x, l = threadIdx.x;
-  const double* B = S + (int64_t)diag[a] * NE;
+  const TS* B = S + (int64_t)diag[a] * NE;
   for (int e = l; e < NA; e += 64) {
     const int i = e / W, j = e % W;
-    A[i][j] = j < ND ? B[i * ND + j] : (j - ND == i ? 1.0 : 0.0);
+    A[i][j] = j < ND ? (double)B[i * ND + j] : (j - ND == i ? 1.0 : 0.0);
   }
   __syncthreads();
   bool bad = false;
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(64) precond_kernel(int32_t n, const int32_t* _
     const int i = e / ND, j = e % ND;
     double v;
     if (bad) {
-      const double d = fabs(B[i * ND + i]);
+      const double d = fabs((double)B[i * ND + i]);
       v = i == j ? 1.0 / (d > 0 ? d : 1.0) : 0.0;
     } else {
       v = A[i][ND + j];
@@ -141,11 +141,12 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(int32_t n, const double* 
 }
 
 // q_a = sum_b S_ab p_b for row a, lanes split as G groups of ND (group g takes every G-th block
-// of the row, lane i of a group the block's row i); groups are summed with shuffles.
-template <int ND>
+// of the row, lane i of a group the block's row i); groups are summed with shuffles. S is read as
+// TS (FP32 in the FP32 form) and widened; the sums are FP64.
+template <int ND, typename TS>
 __device__ __forceinline__ double row_times(const int32_t* __restrict__ row_ptr,
                                             const int32_t* __restrict__ col,
-                                            const double* __restrict__ S, const double* z,
+                                            const TS* __restrict__ S, const double* z,
                                             const double* p_old, double beta, int a, int lane) {
   constexpr int G = 64 / ND, NE = ND * ND;
   const int g = lane / ND, i = lane - g * ND;
@@ -154,11 +155,11 @@ __device__ __forceinline__ double row_times(const int32_t* __restrict__ row_ptr,
     const int t1 = row_ptr[a + 1];
     for (int t = row_ptr[a] + g; t < t1; t += G) {
       const int bc = col[t];
-      const double* Sr = S + (int64_t)t * NE + i * ND;
+      const TS* Sr = S + (int64_t)t * NE + i * ND;
       const double* zb = z + (int64_t)bc * ND;
       const double* pb = p_old + (int64_t)bc * ND;
 #pragma unroll
-      for (int j = 0; j < ND; ++j) acc += Sr[j] * (zb[j] + beta * pb[j]);
+      for (int j = 0; j < ND; ++j) acc += (double)Sr[j] * (zb[j] + beta * pb[j]);
     }
   }
   double tot = acc;
@@ -167,10 +168,10 @@ __device__ __forceinline__ double row_times(const int32_t* __restrict__ row_ptr,
   return tot;  // valid in lanes < ND
 }
 
-template <int ND>
+template <int ND, typename TS>
 __global__ void __launch_bounds__(256) pcg_spmv_kernel(
     int slot, int max_iters, int32_t n, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, const double* __restrict__ S, const double* __restrict__ z,
+    const int32_t* __restrict__ col, const TS* __restrict__ S, const double* __restrict__ z,
     const double* __restrict__ p_old, double* __restrict__ p_new, double* __restrict__ q,
     const double* __restrict__ partB, int nbB, double* __restrict__ partA,
     double* __restrict__ scal, double tol2) {
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(
   }
   // k == max_iters only checks the last residual
   if (conv || !finite || a >= n || k == max_iters) return;
-  const double qi = row_times<ND>(row_ptr, col, S, z, p_old, beta, a, lane);
+  const double qi = row_times<ND, TS>(row_ptr, col, S, z, p_old, beta, a, lane);
   double pq = 0;
   if (lane < ND) {
     const int64_t e = (int64_t)a * ND + lane;
@@ -262,16 +263,16 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(
   }
 }
 
-template <int ND>
+template <int ND, typename TS>
 __global__ void __launch_bounds__(256) bspmv_kernel(int32_t n, const int32_t* __restrict__ row_ptr,
                                                     const int32_t* __restrict__ col,
-                                                    const double* __restrict__ S,
+                                                    const TS* __restrict__ S,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (a >= n) return;
-  const double v = row_times<ND>(row_ptr, col, S, x, x, 0.0, a, lane);
+  const double v = row_times<ND, TS>(row_ptr, col, S, x, x, 0.0, a, lane);
   if (lane < ND) y[(int64_t)a * ND + lane] = v;
 }
 
@@ -293,7 +294,7 @@ BlockPcg::~BlockPcg() {
 
 void BlockPcg::setup(int nd, int32_t n_nodes, const std::vector<int32_t>& row_ptr,
                      const std::vector<int32_t>& col, const std::vector<int32_t>& diag,
-                     hipStream_t s) {
+                     hipStream_t s, bool f32) {
   SYMHIP_CHECK((int)row_ptr.size() == n_nodes + 1 && (int)diag.size() == n_nodes,
                "BlockPcg: pattern size mismatch");
   SYMHIP_CHECK(row_ptr.back() == (int32_t)col.size(), "BlockPcg: row_ptr does not cover col");
@@ -312,7 +313,9 @@ void BlockPcg::setup(int nd, int32_t n_nodes, const std::vector<int32_t>& row_pt
   col_.upload(col, s);
   diag_.upload(diag, s);
   const size_t nv = std::max<size_t>(1, (size_t)n_ * nd);
-  S_.resize(std::max<size_t>(1, (size_t)n_blk_ * nd * nd));
+  f32_ = f32;
+  if (f32_) S32_.resize(std::max<size_t>(1, (size_t)n_blk_ * nd * nd));
+  else S_.resize(std::max<size_t>(1, (size_t)n_blk_ * nd * nd));
   Minv_.resize(std::max<size_t>(1, (size_t)n_ * nd * nd));
   r_.resize(nv);
   z_.resize(nv);
@@ -329,8 +332,12 @@ void BlockPcg::setup(int nd, int32_t n_nodes, const std::vector<int32_t>& row_pt
 void BlockPcg::build_preconditioner(const double* correction, hipStream_t s) {
   if (n_ == 0) return;
   dispatch_nd(nd_, [&]<int ND>() {
-    hipLaunchKernelGGL((precond_kernel<ND>), dim3(n_), dim3(64), 0, s, n_, diag_.get(), S_.get(),
-                       correction, Minv_.get());
+    if (f32_)
+      hipLaunchKernelGGL((precond_kernel<ND, float>), dim3(n_), dim3(64), 0, s, n_, diag_.get(),
+                         S32_.get(), correction, Minv_.get());
+    else
+      hipLaunchKernelGGL((precond_kernel<ND, double>), dim3(n_), dim3(64), 0, s, n_, diag_.get(),
+                         S_.get(), correction, Minv_.get());
   });
   HIP_OK(hipGetLastError());
 }
@@ -338,7 +345,7 @@ void BlockPcg::build_preconditioner(const double* correction, hipStream_t s) {
 void BlockPcg::multiply_with(const double* S, const double* x, double* y, hipStream_t s) {
   if (n_ == 0) return;
   dispatch_nd(nd_, [&]<int ND>() {
-    hipLaunchKernelGGL((bspmv_kernel<ND>), dim3(grid_for(n_, 4)), dim3(256), 0, s, n_,
+    hipLaunchKernelGGL((bspmv_kernel<ND, double>), dim3(grid_for(n_, 4)), dim3(256), 0, s, n_,
                        row_ptr_.get(), col_.get(), S, x, y);
   });
   HIP_OK(hipGetLastError());
@@ -353,10 +360,14 @@ void BlockPcg::capture_chunk(double* x, int max_iters, double tol2, hipStream_t 
   dispatch_nd(nd_, [&]<int ND>() {
     for (int i = 0; i < kChunk; ++i) {
       const int slot = i & 1;
-      hipLaunchKernelGGL((pcg_spmv_kernel<ND>), dim3(grid_for(n_, 4)), dim3(256), 0, s, slot,
-                         max_iters, n_, row_ptr_.get(), col_.get(), S_.get(), z_.get(),
-                         p_[slot ^ 1].get(), p_[slot].get(), q_.get(), partB_.get(),
-                         n_upd_blocks_, partA_.get(), scal_.get(), tol2);
+      auto spmv = [&](auto* S) {
+        using TS = std::remove_pointer_t<decltype(S)>;
+        hipLaunchKernelGGL((pcg_spmv_kernel<ND, TS>), dim3(grid_for(n_, 4)), dim3(256), 0, s,
+                           slot, max_iters, n_, row_ptr_.get(), col_.get(), S, z_.get(),
+                           p_[slot ^ 1].get(), p_[slot].get(), q_.get(), partB_.get(),
+                           n_upd_blocks_, partA_.get(), scal_.get(), tol2);
+      };
+      if (f32_) spmv(S32_.get()); else spmv(S_.get());
       hipLaunchKernelGGL((pcg_update_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, slot,
                          max_iters, n_, Minv_.get(), p_[slot].get(), q_.get(), partA_.get(), x,
                          r_.get(), z_.get(), partB_.get(), scal_.get());

@@ -32,17 +32,27 @@ struct ObsView {
   const int32_t* node;      // [n_obs]
 };
 
+// Per-observation stride of Z (ND x 3 row-major in TW) rounded up to whole 16-byte vectors, so
+// every observation's Z starts 16-byte aligned (the pad entries stay zero).
+template <int ND, typename TW>
+constexpr int z_stride() {
+  constexpr int v = 16 / (int)sizeof(TW);
+  return (ND * 3 + v - 1) / v * v;
+}
+
 // Per observation (thread): evaluate the factor, write its residual and W_o = J_node^T J_point
-// (ND x 3 row-major), and add its J_point^T J_point (lower 6: 00 10 11 20 21 22) and
+// (ND x 3 row-major, stored as TW), and add its J_point^T J_point (lower 6: 00 10 11 20 21 22) and
 // J_point^T r to the point's V_p and g_p. Observations are point-major, so a point's terms sit in
 // consecutive lanes: a segmented shuffle scan sums them inside the wavefront and the last lane of
 // each segment adds the sum with one atomic per value (V and g start at zero).
-template <int FTC, int ND>
+// TW = float is the FP32 form: the Hessian-side products (W, V) come from the FP32 factor body;
+// the residual and the gradient J_point^T r stay FP64.
+template <int FTC, int ND, typename TW>
 __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
                                                          const int32_t* __restrict__ obs_point,
                                                          ObsView ov, const double* __restrict__ values,
                                                          double* __restrict__ residual,
-                                                         double* __restrict__ W,
+                                                         TW* __restrict__ W,
                                                          double* __restrict__ V,
                                                          double* __restrict__ gp) {
   using F = FT<FTC>;
@@ -62,30 +72,43 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
     const int ro = ov.res_off[o];
 #pragma unroll
     for (int i = 0; i < F::D; ++i) residual[ro + i] = r[i];
-    double* Wo = W + (int64_t)o * ND * 3;
+    TW* Wo = W + (int64_t)o * ND * 3;
+    // Hessian-side products in the storage precision
+    auto hess = [&](const auto (&Jh)[F::D][F::T]) {
+      using S = std::remove_cv_t<std::remove_reference_t<decltype(Jh[0][0])>>;
 #pragma unroll
-    for (int i = 0; i < ND; ++i)
+      for (int i = 0; i < ND; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double h = 0;
+        for (int k = 0; k < 3; ++k) {
+          S h = 0;
 #pragma unroll
-        for (int q = 0; q < F::D; ++q) h += J[q][i] * J[q][ND + k];
-        Wo[i * 3 + k] = h;
-      }
-    int idx = 0;
+          for (int q = 0; q < F::D; ++q) h += Jh[q][i] * Jh[q][ND + k];
+          Wo[i * 3 + k] = (TW)h;
+        }
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+          S h = 0;
+#pragma unroll
+          for (int q = 0; q < F::D; ++q) h += Jh[q][ND + i] * Jh[q][ND + j];
+          acc[idx++] = (double)h;
+        }
+    };
+    if constexpr (std::is_same_v<TW, float>) {
+      float rf[F::D], Jf[F::D][F::T];
+      F::template eval<float>(a, rf, Jf);
+      hess(Jf);
+    } else {
+      hess(J);
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double gi = 0;
 #pragma unroll
       for (int q = 0; q < F::D; ++q) gi += J[q][ND + i] * r[q];
       acc[6 + i] = gi;
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        double h = 0;
-#pragma unroll
-        for (int q = 0; q < F::D; ++q) h += J[q][ND + i] * J[q][ND + j];
-        acc[idx++] = h;
-      }
     }
   }
   // inclusive segmented scan over lanes with equal p (segments are contiguous)
@@ -288,24 +311,25 @@ __global__ void __launch_bounds__(256) point_factor_kernel(int32_t n_points,
 // Per (observation, camera row): Z_o(i, :) = W_o(i, :) L_p^-T (forward substitution with L_p).
 // Consecutive threads walk consecutive rows of W and Z, so the 216 B per observation move as
 // contiguous runs.
-template <int ND>
+template <int ND, typename TW>
 __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
                                                          const int32_t* __restrict__ obs_point,
                                                          const double* __restrict__ Lp,
-                                                         const double* __restrict__ W,
-                                                         double* __restrict__ Z) {
+                                                         const TW* __restrict__ W,
+                                                         TW* __restrict__ Z) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)n_obs * ND) return;
   const int64_t o = t / ND;
   const double* L = Lp + (int64_t)obs_point[o] * 6;
-  const double* w = W + t * 3;
-  const double z0 = w[0] / L[0];
-  const double z1 = (w[1] - L[1] * z0) / L[2];
-  const double z2 = (w[2] - L[3] * z0 - L[4] * z1) / L[5];
-  double* z = Z + t * 3;
-  z[0] = z0;
-  z[1] = z1;
-  z[2] = z2;
+  const TW* w = W + t * 3;
+  constexpr int ZS = z_stride<ND, TW>();
+  const double z0 = (double)w[0] / L[0];
+  const double z1 = ((double)w[1] - L[1] * z0) / L[2];
+  const double z2 = ((double)w[2] - L[3] * z0 - L[4] * z1) / L[5];
+  TW* z = Z + o * ZS + (t - o * ND) * 3;
+  z[0] = (TW)z0;
+  z[1] = (TW)z1;
+  z[2] = (TW)z2;
 }
 
 // One wavefront per S block (a, b): S = [Bdiag + damping - ZZ_a if a == b] + Bfac
@@ -317,39 +341,59 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // register r (the FP64 16x16x4 output interleaves rows across the four lane groups). The eight
 // pairs' observation indices are wave-uniform scalar loads (the pair list is padded so a batch
 // reaching past the last block stays inside it). Diagonal blocks carry no pairs: their
-// sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take blocks from a host-built
-// item list (-1 = idle wave). Output: into the LDL^T panels at (base + col * ld + row) or
-// transposed, or (sfull != null) into full symmetric block storage, block row-major, at
-// f_lo[blk] and its mirror f_up[blk] (transposed).
-template <int ND>
+// sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take work items from a
+// host-built list: (block, pair range, flags), blocks with many pairs split into several items
+// whose partial sums are added atomically (kRcsAtomic; the output is zero-filled first) and only
+// the first item of a block adds the B / damping / ZZ terms (kRcsFirst); block -1 = idle wave.
+// Output: into the LDL^T panels at (base + col * ld + row) or transposed, or (sfull != null)
+// into full symmetric block storage, block row-major, at f_lo[blk] and its mirror f_up[blk]
+// (transposed). Z is read as TW (FP32 in the FP32 form) and widened: the sums are FP64 on the
+// matrix cores either way; the full storage is written as TS.
+constexpr int kRcsChunk = 512;  // pairs per work item at most
+constexpr bool kRcsHeaviestFirst = true;  // order each XCD's items by size (else by camera row)
+constexpr int kRcsFirst = 1, kRcsAtomic = 2;
+template <int ND, typename TW, typename TS>
 __global__ void __launch_bounds__(256) rcs_assemble_kernel(
-    int32_t n_items, const int32_t* __restrict__ items, const int32_t* __restrict__ blk_a,
-    const int32_t* __restrict__ blk_b, const int32_t* __restrict__ pair_ptr,
-    const int32_t* __restrict__ pairs, const double* __restrict__ Z,
+    int32_t n_items, const int4* __restrict__ items, const int32_t* __restrict__ blk_a,
+    const int32_t* __restrict__ blk_b, const int32_t* __restrict__ pairs,
+    const TW* __restrict__ Z,
     const double* __restrict__ ZZ, const double* __restrict__ Bdiag,
     const double* __restrict__ Bfac, const double* __restrict__ dnode,
     const int64_t* __restrict__ panel_base, const int32_t* __restrict__ panel_ld,
     const int32_t* __restrict__ panel_trans, double* __restrict__ panel,
-    double* __restrict__ sfull, const int32_t* __restrict__ f_lo,
+    TS* __restrict__ sfull, const int32_t* __restrict__ f_lo,
     const int32_t* __restrict__ f_up) {
   static_assert(ND <= 16, "camera node wider than the MFMA tile");
   typedef double d4 __attribute__((ext_vector_type(4)));
-  constexpr int NE = ND * ND, NZ = ND * 3;
+  constexpr int NE = ND * ND;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n_items) return;
-  const int blk = __builtin_amdgcn_readfirstlane(items[t]);
+  const int4 item = items[t];
+  const int blk = __builtin_amdgcn_readfirstlane(item.x);
   if (blk < 0) return;
+  const int q_begin = __builtin_amdgcn_readfirstlane(item.y);
+  const int q_end = __builtin_amdgcn_readfirstlane(item.z);
+  const int flags = __builtin_amdgcn_readfirstlane(item.w);
   const int lane = threadIdx.x & 63;
   const int a = blk_a[blk], b = blk_b[blk];
   const int row = lane & 15, kg = lane >> 4;
   const bool live = row < ND;
   const int rowoff = (live ? row : 0) * 3;
   d4 acc0 = {}, acc1 = {}, acc2 = {};
-  const int q_begin = pair_ptr[blk], q_end = pair_ptr[blk + 1];
   // K entry 4h + kg of a batch belongs to its pair p = (4h + kg) / 3 (column (4h + kg) % 3), which
-  // is pair plo = 4h / 3 or plo + 1: a select between two wave-uniform indices.
-  auto load_z = [&](int q, const int (&xs)[8], const int (&ys)[8], double (&av)[6],
-                    double (&bv)[6]) {
+  // is pair plo = 4h / 3 or plo + 1: a select between two wave-uniform indices. Indices past the
+  // item's end come from the next pairs or the padding (valid observations), masked out by ok.
+  // Enough waves stay resident (about 110 registers each) to overlap one wave's loads with the
+  // others' MFMAs.
+  constexpr int ZS = z_stride<ND, TW>();
+  for (int q = q_begin; q < q_end; q += 8) {
+    int xs[8], ys[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      xs[p] = pairs[2 * (q + p)];
+      ys[p] = pairs[2 * (q + p) + 1];
+    }
+    double av[6], bv[6];
 #pragma unroll
     for (int h = 0; h < 6; ++h) {
       const int K = 4 * h + kg;
@@ -359,24 +403,11 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
       const int x = hi ? xs[plo + 1] : xs[plo];
       const int y = hi ? ys[plo + 1] : ys[plo];
       const bool ok = live && q + p < q_end;
-      const double za = Z[(int64_t)x * NZ + rowoff + col];
-      const double zb = Z[(int64_t)y * NZ + rowoff + col];
+      const double za = (double)Z[(int64_t)x * ZS + rowoff + col];
+      const double zb = (double)Z[(int64_t)y * ZS + rowoff + col];
       av[h] = ok ? za : 0.0;
       bv[h] = ok ? zb : 0.0;
     }
-  };
-  // Indices past the block's end come from the next block or the padding (valid observations),
-  // masked out by ok. Enough waves stay resident (about 90 registers each) to overlap one wave's
-  // loads with the others' MFMAs.
-  for (int q = q_begin; q < q_end; q += 8) {
-    int xs[8], ys[8];
-#pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      xs[p] = pairs[2 * (q + p)];
-      ys[p] = pairs[2 * (q + p) + 1];
-    }
-    double av[6], bv[6];
-    load_z(q, xs, ys, av, bv);
     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], acc0, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], acc1, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[2], bv[2], acc2, 0, 0, 0);
@@ -392,31 +423,42 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     const int i = kg + 4 * r;
     if (i >= ND) continue;
     if (a == b && i < j) continue;
-    double s = Bfac[(int64_t)blk * NE + i * ND + j] - sum[r];
-    if (a == b)
-      s += Bdiag[(int64_t)a * NE + i * ND + j] - ZZ[(int64_t)a * NE + i * ND + j] +
-           (i == j ? dnode[a * ND + i] : 0.0);
+    double s = -sum[r];
+    if (flags & kRcsFirst) {
+      s += Bfac[(int64_t)blk * NE + i * ND + j];
+      if (a == b)
+        s += Bdiag[(int64_t)a * NE + i * ND + j] - ZZ[(int64_t)a * NE + i * ND + j] +
+             (i == j ? dnode[a * ND + i] : 0.0);
+    }
+    const bool atomic = flags & kRcsAtomic;
     if (sfull) {
-      sfull[(int64_t)f_lo[blk] * NE + i * ND + j] = s;
-      sfull[(int64_t)f_up[blk] * NE + j * ND + i] = s;
+      TS* lo = sfull + (int64_t)f_lo[blk] * NE + i * ND + j;
+      TS* up = sfull + (int64_t)f_up[blk] * NE + j * ND + i;
+      if (atomic) {
+        unsafeAtomicAdd(lo, (TS)s);
+        unsafeAtomicAdd(up, (TS)s);
+      } else {
+        *lo = (TS)s;
+        *up = (TS)s;
+      }
     } else {
       // entry (row a*ND+i, col b*ND+j) of S
       const int64_t base = panel_base[blk];
       const int ld = panel_ld[blk];
       const int64_t pos = panel_trans[blk] ? base + (int64_t)i * ld + j : base + (int64_t)j * ld + i;
-      panel[pos] = s;
+      if (atomic) unsafeAtomicAdd(panel + pos, s); else panel[pos] = s;
     }
   }
 }
 
 // Per camera node (wavefront): v_a = g_a - sum_{o in a} Z_o h_p(o) and the diagonal-block
 // elimination term ZZ_a = sum_{o in a} Z_o Z_o^T (full ND x ND, row-major).
-template <int ND>
+template <int ND, typename TW>
 __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           const int32_t* __restrict__ cam_obs_ptr,
                                                           const int32_t* __restrict__ cam_obs,
                                                           const int32_t* __restrict__ obs_point,
-                                                          const double* __restrict__ Z,
+                                                          const TW* __restrict__ Z,
                                                           const double* __restrict__ hp,
                                                           const double* __restrict__ gB,
                                                           double* __restrict__ v,
@@ -430,10 +472,10 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
     const int o = cam_obs[q];
     const int p = obs_point[o];
     const double h0 = hp[p * 3], h1 = hp[p * 3 + 1], h2 = hp[p * 3 + 2];
-    const double* Zo = Z + (int64_t)o * ND * 3;
+    const TW* Zo = Z + (int64_t)o * z_stride<ND, TW>();
     double z[ND * 3];
 #pragma unroll
-    for (int e = 0; e < ND * 3; ++e) z[e] = Zo[e];
+    for (int e = 0; e < ND * 3; ++e) z[e] = (double)Zo[e];
 #pragma unroll
     for (int i = 0; i < ND; ++i) acc[i] += z[i * 3] * h0 + z[i * 3 + 1] * h1 + z[i * 3 + 2] * h2;
     int idx = 0;
@@ -467,22 +509,22 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
 
 // Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
 // update vector in reference state order.
-template <int ND>
+template <int ND, typename TW>
 __global__ void __launch_bounds__(128) back_substitute_kernel(
     int32_t n_points, const int32_t* __restrict__ pt_obs_ptr, const int32_t* __restrict__ obs_node,
-    const double* __restrict__ Z, const double* __restrict__ Lp, const double* __restrict__ hp,
+    const TW* __restrict__ Z, const double* __restrict__ Lp, const double* __restrict__ hp,
     const double* __restrict__ y, const int32_t* __restrict__ pt_tan, double* __restrict__ update) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_points) return;
   double t0 = hp[p * 3], t1 = hp[p * 3 + 1], t2 = hp[p * 3 + 2];
   for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
-    const double* Zo = Z + (int64_t)o * ND * 3;
+    const TW* Zo = Z + (int64_t)o * z_stride<ND, TW>();
     const double* yn = y + obs_node[o] * ND;
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
-      t0 -= Zo[i * 3] * yn[i];
-      t1 -= Zo[i * 3 + 1] * yn[i];
-      t2 -= Zo[i * 3 + 2] * yn[i];
+      t0 -= (double)Zo[i * 3] * yn[i];
+      t1 -= (double)Zo[i * 3 + 1] * yn[i];
+      t2 -= (double)Zo[i * 3 + 2] * yn[i];
     }
   }
   const double* L = Lp + (int64_t)p * 6;
@@ -573,7 +615,9 @@ class SchurBackend final : public SystemBackend {
   // observation tables
   DBuf<int32_t> obs_arg_off_, obs_res_off_, obs_node_, obs_point_, pt_obs_ptr_, cam_obs_ptr_,
       cam_obs_;
-  DBuf<int32_t> node_tan_, pt_tan_, blk_a_, blk_b_, pair_ptr_, pairs_, diag_blk_, rcs_items_;
+  DBuf<int32_t> node_tan_, pt_tan_, blk_a_, blk_b_, pair_ptr_, pairs_, diag_blk_;
+  DBuf<uint8_t> rcs_items_;  // int4 work items of the assembly kernel
+  int32_t n_rcs_items_ = 0;
   DBuf<int64_t> panel_base_;
   DBuf<int32_t> panel_ld_, panel_trans_;
   // camera-only factors
@@ -586,6 +630,18 @@ class SchurBackend final : public SystemBackend {
   // per state block storage
   DBuf<double> residual_[3], W_[3], V_[3], gp_[3], Bdiag_[3], Bfac_[3], gB_[3], rhs_[3];
   DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, ZZ_, v_, y_, partials_;
+  // FP32 form (precision 1): while PCG runs, W and Z are stored in FP32 (W32_, Z32_) and so is S
+  // for PCG; residual, gradient, sums, LDL^T and the LM decisions stay FP64. Once the run has
+  // switched to the exact LDL^T (exact_now_), linearization returns to plain FP64 storage, the
+  // FP64 method finishing the run as its judge. w_f32_[b]: precision of block b's W.
+  bool f32_ = false;
+  bool w_f32_[3] = {false, false, false};
+  DBuf<float> W32_[3], Z32_;
+  // Calls f(W, Z) with the typed W of state block b and the Z storage of the same precision.
+  template <typename F>
+  void with_storage(int b, F&& f) {
+    if (w_f32_[b]) f(W32_[b].get(), Z32_.get()); else f(W_[b].get(), Z_.get());
+  }
   SupernodalLDLT chol_;
   std::unique_ptr<CscLinearizer> csc_;  // reference-layout H for best-linearization output
   // Iterative form: PCG on the reduced camera system in full block storage, falling back to the
@@ -622,6 +678,9 @@ struct UnionFind {
 
 SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStream_t s)
     : host_(&P), stream_(s) {
+  SYMHIP_CHECK(o.precision == SYMHIP_PRECISION_F64 || o.precision == SYMHIP_PRECISION_F32_JAC,
+               "symhip: unknown precision");
+  f32_ = o.precision == SYMHIP_PRECISION_F32_JAC;
   const int32_t S = (int32_t)P.state_keys.size();
   N_ = P.N;
   M_ = P.M;
@@ -842,9 +901,11 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   }
   // Assembly work list. Workgroups are dispatched round-robin over the XCDs (XCD = workgroup
   // index mod kXcds), each with its own L2: XCD x takes a contiguous range of camera rows holding
-  // about 1/kXcds of the pairs (blocks of one row share the row camera's Z), heaviest blocks
-  // first so the longest waves start early; -1 pads the XCDs to equal length.
-  std::vector<int32_t> rcs_items;
+  // about 1/kXcds of the pairs. Blocks with more than kRcsChunk pairs become several items, and
+  // each XCD's items run largest first, so the longest waves start early (row order keeps more of
+  // Z in L2, a quarter of the traffic, but measured slower: the kernel is latency-bound, not
+  // HBM-bound); -1 pads the XCDs to equal length.
+  std::vector<int4> rcs_items;
   {
     std::vector<int64_t> before(n_nodes_ + 1, 0);  // pairs (+1 per block) in rows < a
     for (int32_t k = 0; k < n_blocks_; ++k) before[blk_a[k] + 1] += pcount[k + 1] - pcount[k] + 1;
@@ -852,18 +913,27 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     std::vector<std::vector<int32_t>> per(kXcds);
     for (int32_t k = 0; k < n_blocks_; ++k)
       per[std::min<int64_t>(kXcds - 1, before[blk_a[k]] * kXcds / before[n_nodes_])].push_back(k);
-    size_t len = 0;
-    for (auto& v : per) {
-      std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
-        return pcount[x + 1] - pcount[x] > pcount[y + 1] - pcount[y];
-      });
-      len = std::max(len, v.size());
-    }
-    len = (len + 3) / 4 * 4;
-    rcs_items.assign(kXcds * len, -1);
+    // blocks with more than kRcsChunk pairs become several items (no wave carries a long tail)
+    std::vector<std::vector<int4>> its(kXcds);
     for (int x = 0; x < kXcds; ++x)
-      for (size_t i = 0; i < per[x].size(); ++i)
-        rcs_items[((i / 4) * kXcds + x) * 4 + i % 4] = per[x][i];
+      for (int32_t k : per[x]) {
+        const int32_t q0 = pcount[k], q1 = pcount[k + 1];
+        const int32_t nc = std::max<int32_t>(1, (q1 - q0 + kRcsChunk - 1) / kRcsChunk);
+        for (int32_t c = 0; c < nc; ++c)
+          its[x].push_back(make_int4(k, q0 + c * kRcsChunk, std::min(q1, q0 + (c + 1) * kRcsChunk),
+                                     (c == 0 ? kRcsFirst : 0) | (nc > 1 ? kRcsAtomic : 0)));
+      }
+    if (kRcsHeaviestFirst)
+      for (auto& v : its)
+        std::stable_sort(v.begin(), v.end(),
+                         [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+    size_t len = 0;
+    for (auto& v : its) len = std::max(len, v.size());
+    len = (len + 3) / 4 * 4;
+    rcs_items.assign(kXcds * len, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < kXcds; ++x)
+      for (size_t i = 0; i < its[x].size(); ++i)
+        rcs_items[((i / 4) * kXcds + x) * 4 + i % 4] = its[x][i];
   }
   // RCS symbolic: nodes = cameras (uniform nd), lower CSC pattern of the dense blocks.
   const int32_t nB = n_nodes_ * nd_;
@@ -947,13 +1017,15 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
       fup[k] = full_index(blk_b[k], blk_a[k]);
     }
     for (int32_t a = 0; a < n_nodes_; ++a) fdiag[a] = full_index(a, a);
-    pcg_.setup(nd_, n_nodes_, frp, fcol, fdiag, s);
+    pcg_.setup(nd_, n_nodes_, frp, fcol, fdiag, s, f32_);
     full_row_ptr_ = frp;
     full_col_ = fcol;
     SYMHIP_CHECK(o.pcg_preconditioner == 0 || o.pcg_preconditioner == 1,
                  "unknown pcg_preconditioner");
     if (nd_ == LearnedPrecond::kNd) gnn_.setup(n_nodes_, frp, fcol, fdiag, s);
     use_learned_ = o.pcg_preconditioner == 1;
+    SYMHIP_CHECK(!use_learned_ || !f32_,
+                 "the learned preconditioner works on FP64 systems (precision 0)");
     SYMHIP_CHECK(!use_learned_ || gnn_.ready(),
                  "the learned preconditioner needs 9-dof camera nodes");
     f_lo_.upload(flo, s);
@@ -1007,7 +1079,10 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   blk_b_.upload(blk_b, s);
   pair_ptr_.upload(pcount, s);
   pairs_.upload(pairs, s);
-  rcs_items_.upload(rcs_items, s);
+  n_rcs_items_ = (int32_t)rcs_items.size();
+  rcs_items_.resize(rcs_items.size() * sizeof(int4));
+  HIP_OK(hipMemcpyAsync(rcs_items_.get(), rcs_items.data(), rcs_items.size() * sizeof(int4),
+                        hipMemcpyHostToDevice, s));
   diag_blk_.upload(diag_blk, s);
   panel_base_.upload(pbase, s);
   panel_ld_.upload(pld, s);
@@ -1016,6 +1091,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   for (int b = 0; b < 3; ++b) {
     residual_[b].resize(std::max(1, M_));
     W_[b].resize((size_t)n_obs_ * nd_ * 3);
+    if (f32_) W32_[b].resize((size_t)n_obs_ * nd_ * 3);
     V_[b].resize((size_t)n_points_ * 6);
     gp_[b].resize((size_t)n_points_ * 3);
     Bdiag_[b].resize((size_t)n_nodes_ * nde);
@@ -1029,7 +1105,18 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   dpoint_.resize((size_t)n_points_ * 3);
   Lp_.resize((size_t)n_points_ * 6);
   hp_.resize((size_t)n_points_ * 3);
-  Z_.resize((size_t)n_obs_ * nd_ * 3);
+  {
+    // Z rows padded to 16-byte multiples (z_stride); the pad entries are zeroed here and never
+    // written again
+    const size_t zs64 = nd_ == 9 ? z_stride<9, double>() : z_stride<6, double>();
+    const size_t zs32 = nd_ == 9 ? z_stride<9, float>() : z_stride<6, float>();
+    Z_.resize((size_t)n_obs_ * zs64);
+    fill_zero(Z_.get(), (int64_t)Z_.size(), s);
+    if (f32_) {
+      Z32_.resize((size_t)n_obs_ * zs32);
+      fill_zero(Z32_.get(), (int64_t)Z32_.size(), s);
+    }
+  }
   ZZ_.resize((size_t)n_nodes_ * nde);
   v_.resize((size_t)n_nodes_ * nd_);
   y_.resize((size_t)n_nodes_ * nd_);
@@ -1046,9 +1133,13 @@ void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
         PhaseScope ps(timer_, SYMHIP_PHASE_LIN_POINTS, s);
         fill_zero(V_[b].get(), (int64_t)n_points_ * 6, s);
         fill_zero(gp_[b].get(), (int64_t)n_points_ * 3, s);
-        hipLaunchKernelGGL((lin_points_kernel<C, ND>), dim3(grid_for(n_obs_, 256)), dim3(256), 0,
-                           s, n_obs_, obs_point_.get(), ov, values, residual_[b].get(),
-                           W_[b].get(), V_[b].get(), gp_[b].get());
+        w_f32_[b] = f32_ && !exact_now_;
+        with_storage(b, [&](auto* W, auto*) {
+          using TW = std::remove_pointer_t<decltype(W)>;
+          hipLaunchKernelGGL((lin_points_kernel<C, ND, TW>), dim3(grid_for(n_obs_, 256)),
+                             dim3(256), 0, s, n_obs_, obs_point_.get(), ov, values,
+                             residual_[b].get(), W, V_[b].get(), gp_[b].get());
+        });
       }
       PhaseScope ps(timer_, SYMHIP_PHASE_LIN_CAMS, s);
       hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
@@ -1097,19 +1188,35 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     PhaseScope ps(timer_, SYMHIP_PHASE_POINT_ELIM, s);
     hipLaunchKernelGGL(point_factor_kernel, dim3(grid_for(n_points_, 256)), dim3(256), 0, s,
                        n_points_, V_[b].get(), dpoint_.get(), gp_[b].get(), Lp_.get(), hp_.get());
-    hipLaunchKernelGGL((point_elim_kernel<ND>), dim3(grid_for((int64_t)n_obs_ * ND, 256)),
-                       dim3(256), 0, s, n_obs_, obs_point_.get(), Lp_.get(), W_[b].get(), Z_.get());
+    with_storage(b, [&](auto* W, auto* Z) {
+      using TW = std::remove_pointer_t<decltype(W)>;
+      hipLaunchKernelGGL((point_elim_kernel<ND, TW>), dim3(grid_for((int64_t)n_obs_ * ND, 256)),
+                         dim3(256), 0, s, n_obs_, obs_point_.get(), Lp_.get(), W, Z);
+    });
   }
   auto assemble = [&](bool full) {
+    // split blocks accumulate atomically: the target storage starts at zero
+    const int64_t nfull = pcg_.n_blocks() * ND * ND;
     if (!full) fill_zero(chol_.panels(), chol_.symbolic().panel_size(), s);
+    else if (f32_) fill_zero(pcg_.blocks32(), nfull, s);
+    else fill_zero(pcg_.blocks(), nfull, s);
     PhaseScope ps(timer_, SYMHIP_PHASE_RCS_ASSEMBLE, s);  // exactly the one assembly kernel
-    const int32_t n_items = (int32_t)rcs_items_.size();
-    hipLaunchKernelGGL((rcs_assemble_kernel<ND>), dim3(n_items / 4), dim3(256), 0, s, n_items,
-                       rcs_items_.get(), blk_a_.get(), blk_b_.get(), pair_ptr_.get(),
-                       pairs_.get(), Z_.get(), ZZ_.get(), Bdiag_[b].get(), Bfac_[b].get(),
-                       dnode_.get(),
-                       panel_base_.get(), panel_ld_.get(), panel_trans_.get(), chol_.panels(),
-                       full ? pcg_.blocks() : nullptr, f_lo_.get(), f_up_.get());
+    const int32_t n_items = n_rcs_items_;
+    auto launch = [&](auto* Z, auto* sfull) {
+      using TW = std::remove_cv_t<std::remove_pointer_t<decltype(Z)>>;
+      using TS = std::remove_pointer_t<decltype(sfull)>;
+      hipLaunchKernelGGL((rcs_assemble_kernel<ND, TW, TS>), dim3(n_items / 4), dim3(256), 0, s,
+                         n_items, reinterpret_cast<const int4*>(rcs_items_.get()), blk_a_.get(),
+                         blk_b_.get(), pairs_.get(), Z, ZZ_.get(), Bdiag_[b].get(),
+                         Bfac_[b].get(), dnode_.get(),
+                         panel_base_.get(), panel_ld_.get(), panel_trans_.get(),
+                         chol_.panels(), sfull, f_lo_.get(), f_up_.get());
+    };
+    // Z in the precision its state block was linearized in; S for PCG in the form's precision
+    with_storage(b, [&](auto*, auto* Z) {
+      if (full && f32_) launch(Z, pcg_.blocks32());
+      else launch(Z, full ? pcg_.blocks() : static_cast<double*>(nullptr));
+    });
   };
   auto direct = [&]() {
     {
@@ -1122,9 +1229,12 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   const bool pcg_now = use_pcg_ && !exact_now_;
   {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
-    hipLaunchKernelGGL((reduced_rhs_kernel<ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
-                       dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
-                       obs_point_.get(), Z_.get(), hp_.get(), gB_[b].get(), v_.get(), ZZ_.get());
+    with_storage(b, [&](auto*, auto* Z) {
+      using TW = std::remove_pointer_t<decltype(Z)>;
+      hipLaunchKernelGGL((reduced_rhs_kernel<ND, TW>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
+                         dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
+                         obs_point_.get(), Z, hp_.get(), gB_[b].get(), v_.get(), ZZ_.get());
+    });
   }
   assemble(pcg_now);
   if (pcg_now && (int)logged_S_.size() < log_max_) {
@@ -1168,9 +1278,12 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   PhaseScope ps(timer_, SYMHIP_PHASE_BACK_SUB, s);
   hipLaunchKernelGGL(scatter_neg_kernel, dim3(grid_for(n_nodes_ * ND, 256)), dim3(256), 0, s,
                      n_nodes_ * ND, node_tan_.get(), y_.get(), update);
-  hipLaunchKernelGGL((back_substitute_kernel<ND>), dim3(grid_for(n_points_, 128)), dim3(128), 0, s,
-                     n_points_, pt_obs_ptr_.get(), obs_node_.get(), Z_.get(), Lp_.get(), hp_.get(),
-                     y_.get(), pt_tan_.get(), update);
+  with_storage(b, [&](auto*, auto* Z) {
+    using TW = std::remove_pointer_t<decltype(Z)>;
+    hipLaunchKernelGGL((back_substitute_kernel<ND, TW>), dim3(grid_for(n_points_, 128)),
+                       dim3(128), 0, s, n_points_, pt_obs_ptr_.get(), obs_node_.get(), Z,
+                       Lp_.get(), hp_.get(), y_.get(), pt_tan_.get(), update);
+  });
   HIP_OK(hipGetLastError());
 }
 
@@ -1218,6 +1331,7 @@ void SchurBackend::panels_from_full(const double* S, hipStream_t s) {
 
 void SchurBackend::log_systems(int max) {
   SYMHIP_CHECK(use_pcg_, "logging reduced camera systems needs use_pcg");
+  SYMHIP_CHECK(!f32_, "logging reduced camera systems needs FP64 systems (precision 0)");
   SYMHIP_CHECK(max >= 0, "max must be non-negative");
   log_max_ = max;
   logged_S_.clear();
