@@ -111,9 +111,10 @@ def run_gpu(args):
 
 def fast_options(args, switch=True, precision=None):
     """The fast form: block-Jacobi PCG on the reduced camera system (truncated at the tolerance or
-    the iteration cap), in the FP32 precision form by default (W, Z and S stored in FP32, FP64
-    sums, cost and decisions), switching to the exact supernodal LDL^T once an accepted step's
-    relative reduction falls below --pcg-exact-below."""
+    the iteration cap; the tolerance tightened with the last accepted step's reduction by
+    --pcg-forcing), in the FP32 precision form by default (W, Z and S stored in FP32, FP64 sums,
+    cost and decisions), switching to the exact supernodal LDL^T (and FP64 storage) once an
+    accepted step's relative reduction falls below --pcg-exact-below."""
     prec = args.fast_precision if precision is None else precision
     return capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
                                ordering=capi.ORDERING_NESTED_DISSECTION,
@@ -121,7 +122,8 @@ def fast_options(args, switch=True, precision=None):
                                capi.PRECISION_F64,
                                use_pcg=1, pcg_max_iters=args.pcg_max_iters,
                                pcg_rel_tol=args.pcg_tol,
-                               pcg_exact_below_reduction=args.pcg_exact_below if switch else 0.0)
+                               pcg_exact_below_reduction=args.pcg_exact_below if switch else 0.0,
+                               pcg_forcing=args.pcg_forcing)
 
 
 def time_to_cost(prob, opts, target):
@@ -236,9 +238,11 @@ def main():
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
     ap.add_argument("--fast-precision", choices=("f32", "f64"), default="f32",
                     help="storage precision of the fast form's W, Z and S")
-    ap.add_argument("--pcg-tol", type=float, default=1e-2)
-    ap.add_argument("--pcg-max-iters", type=int, default=30)
-    ap.add_argument("--pcg-exact-below", type=float, default=3e-4,
+    ap.add_argument("--pcg-tol", type=float, default=3e-2)
+    ap.add_argument("--pcg-max-iters", type=int, default=20)
+    ap.add_argument("--pcg-forcing", type=float, default=0.3,
+                    help="inexact-Newton forcing of the PCG tolerance (0: fixed tolerance)")
+    ap.add_argument("--pcg-exact-below", type=float, default=1e-5,
                     help="switch the fast form to the exact LDL^T below this LM reduction")
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the run-to-convergence (time-to-cost) leg, e.g. under a profiler")
@@ -276,6 +280,7 @@ def main():
                                                  if args.fast_precision == "f32" else "f64"),
                                        "pcg_rel_tol": args.pcg_tol,
                                        "pcg_max_iters": args.pcg_max_iters,
+                                       "pcg_forcing": args.pcg_forcing,
                                        "exact_ldlt_below_reduction": args.pcg_exact_below},
                          "speedup": speedup(r["plain"], r["fast"])},
         "setup_s": r["t_setup"],

@@ -112,6 +112,11 @@ typedef struct symhip_solver_options {
   /* Truncated PCG only: once an accepted LM step's relative reduction falls below this value the
      remaining iterations of the run use the direct LDL^T (0: never). */
   double pcg_exact_below_reduction;
+  /* Truncated PCG only, inexact-Newton forcing (0: off): after an accepted LM step with relative
+     reduction r the next solve uses tolerance min(pcg_rel_tol, pcg_forcing * sqrt(r)) (at least
+     1e-10) and an iteration limit of pcg_max_iters * (1 + 2 log10(pcg_rel_tol / tolerance)), at
+     most 20 * pcg_max_iters. */
+  double pcg_forcing;
 } symhip_solver_options;
 
 enum symhip_pcg_preconditioner {

@@ -344,6 +344,7 @@ class SolverOptions(C.Structure):
         ("pcg_rel_tol", C.c_double),
         ("pcg_preconditioner", C.c_int32),
         ("pcg_exact_below_reduction", C.c_double),
+        ("pcg_forcing", C.c_double),
     ]
 
 
@@ -455,9 +456,9 @@ class HipProblem:
 
 def solver_options(linear_solver=SOLVER_CHOLESKY, ordering=ORDERING_MIN_DEGREE,
                    precision=PRECISION_F64, use_pcg=0, pcg_max_iters=100, pcg_rel_tol=1e-6,
-                   pcg_preconditioner=0, pcg_exact_below_reduction=0.0):
+                   pcg_preconditioner=0, pcg_exact_below_reduction=0.0, pcg_forcing=0.0):
     return SolverOptions(linear_solver, ordering, precision, use_pcg, pcg_max_iters, pcg_rel_tol,
-                         pcg_preconditioner, pcg_exact_below_reduction)
+                         pcg_preconditioner, pcg_exact_below_reduction, pcg_forcing)
 
 
 class HipOptimizer:

@@ -159,6 +159,7 @@ int symhip_optimizer_create(const symhip_problem_desc* desc, const symhip_optimi
       o.pcg_rel_tol = options->pcg_rel_tol;
       o.pcg_preconditioner = options->pcg_preconditioner;
       o.pcg_exact_below_reduction = options->pcg_exact_below_reduction;
+      o.pcg_forcing = options->pcg_forcing;
     }
     SYMHIP_CHECK(params->lambda_update_type == SYMHIP_LAMBDA_STATIC ||
                      params->lambda_update_type == SYMHIP_LAMBDA_DYNAMIC,

@@ -23,6 +23,7 @@ struct SolverOptions {
   double pcg_rel_tol = 1e-6;
   int pcg_preconditioner = 0;  // 0 block Jacobi, 1 learned correction
   double pcg_exact_below_reduction = 0;
+  double pcg_forcing = 0;
 };
 
 // A linear system backend owns the linearization storage of the three LM state blocks and the

@@ -51,11 +51,9 @@ class BlockPcg {
   DBuf<double> S_, Minv_, r_, z_, p_[2], q_, partA_, partB_, scal_;
   DBuf<float> S32_;
   double* host_scal_ = nullptr;  // pinned
-  hipGraphExec_t chunk_ = nullptr;  // kChunk iterations, captured for (x, max_iters, tol)
+  hipGraphExec_t chunk_ = nullptr;  // kChunk iterations, captured for the output vector x
   double* chunk_x_ = nullptr;
-  int chunk_max_ = -1;
-  double chunk_tol2_ = -1;
-  void capture_chunk(double* x, int max_iters, double tol2, hipStream_t s);
+  void capture_chunk(double* x, hipStream_t s);
  public:
   BlockPcg() = default;
   BlockPcg(const BlockPcg&) = delete;

@@ -9,9 +9,10 @@
 // preconditioner z = M r and writes the next partial sums. Scalars live in a small device array:
 //   [0], [1] rz by iteration parity, [2] rr_0, [3] state (0 running, 1 converged,
 //   2 breakdown), [4] iteration count at the stop, [5] last rr, [6], [7] the iteration index k
-//   by slot. The index is read from the device (kernel s of an iteration pair uses slot s and
-//   the spmv kernel writes k + 1 into the other slot), so a chunk of iterations is the same
-//   launch sequence every time and is captured once in a hipGraph and replayed.
+//   by slot, [8] tol^2, [9] the iteration limit. The index is read from the device (kernel s of
+//   an iteration pair uses slot s and the spmv kernel writes k + 1 into the other slot) and the
+//   tolerance and limit are set by the (uncaptured) init kernel, so a chunk of iterations is the
+//   same launch sequence for every solve and is captured once in a hipGraph and replayed.
 #include <algorithm>
 
 #include "./pcg.h"
@@ -21,7 +22,7 @@ namespace symhip {
 
 namespace {
 
-constexpr int kScal = 8;
+constexpr int kScal = 10;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -108,7 +109,8 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(int32_t n, const double* 
                                                        double* __restrict__ z,
                                                        double* __restrict__ p_old,
                                                        double* __restrict__ partB,
-                                                       double* __restrict__ scal) {
+                                                       double* __restrict__ scal, double tol2,
+                                                       int max_iters) {
   constexpr int RPB = 256 / ND, NE = ND * ND;
   __shared__ double rs[RPB * ND];
   const int t = threadIdx.x, rloc = t / ND, i = t - rloc * ND;
@@ -136,7 +138,9 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(int32_t n, const double* 
     partB[2 * blockIdx.x + 1] = rr;
   }
   if (blockIdx.x == 0 && t == 0) {
-    for (int k = 0; k < kScal; ++k) scal[k] = 0.0;
+    for (int k = 0; k < 8; ++k) scal[k] = 0.0;
+    scal[8] = tol2;
+    scal[9] = (double)max_iters;
   }
 }
 
@@ -170,13 +174,15 @@ __device__ __forceinline__ double row_times(const int32_t* __restrict__ row_ptr,
 
 template <int ND, typename TS>
 __global__ void __launch_bounds__(256) pcg_spmv_kernel(
-    int slot, int max_iters, int32_t n, const int32_t* __restrict__ row_ptr,
+    int slot, int32_t n, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const TS* __restrict__ S, const double* __restrict__ z,
     const double* __restrict__ p_old, double* __restrict__ p_new, double* __restrict__ q,
     const double* __restrict__ partB, int nbB, double* __restrict__ partA,
-    double* __restrict__ scal, double tol2) {
+    double* __restrict__ scal) {
   if (scal[3] != 0.0) return;
   const int k = (int)scal[6 + slot];
+  const double tol2 = scal[8];
+  const int max_iters = (int)scal[9];
   if (k > max_iters) return;
   const int lane = threadIdx.x & 63;
   const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -218,13 +224,13 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(
 
 template <int ND>
 __global__ void __launch_bounds__(256) pcg_update_kernel(
-    int slot, int max_iters, int32_t n, const double* __restrict__ Minv,
+    int slot, int32_t n, const double* __restrict__ Minv,
     const double* __restrict__ p_new, const double* __restrict__ q,
     const double* __restrict__ partA, double* __restrict__ x, double* __restrict__ r,
     double* __restrict__ z, double* __restrict__ partB, double* __restrict__ scal) {
   if (scal[3] != 0.0) return;
   const int k = (int)scal[6 + slot];
-  if (k >= max_iters) return;
+  if (k >= (int)scal[9]) return;
   constexpr int RPB = 256 / ND, NE = ND * ND;
   __shared__ double rs[RPB * ND];
   double pq = 0, dummy = 0;
@@ -351,8 +357,8 @@ void BlockPcg::multiply_with(const double* S, const double* x, double* y, hipStr
   HIP_OK(hipGetLastError());
 }
 
-void BlockPcg::capture_chunk(double* x, int max_iters, double tol2, hipStream_t s) {
-  if (chunk_ && chunk_x_ == x && chunk_max_ == max_iters && chunk_tol2_ == tol2) return;
+void BlockPcg::capture_chunk(double* x, hipStream_t s) {
+  if (chunk_ && chunk_x_ == x) return;
   if (chunk_) (void)hipGraphExecDestroy(chunk_);
   chunk_ = nullptr;
   hipGraph_t g;
@@ -363,22 +369,20 @@ void BlockPcg::capture_chunk(double* x, int max_iters, double tol2, hipStream_t 
       auto spmv = [&](auto* S) {
         using TS = std::remove_pointer_t<decltype(S)>;
         hipLaunchKernelGGL((pcg_spmv_kernel<ND, TS>), dim3(grid_for(n_, 4)), dim3(256), 0, s,
-                           slot, max_iters, n_, row_ptr_.get(), col_.get(), S, z_.get(),
+                           slot, n_, row_ptr_.get(), col_.get(), S, z_.get(),
                            p_[slot ^ 1].get(), p_[slot].get(), q_.get(), partB_.get(),
-                           n_upd_blocks_, partA_.get(), scal_.get(), tol2);
+                           n_upd_blocks_, partA_.get(), scal_.get());
       };
       if (f32_) spmv(S32_.get()); else spmv(S_.get());
       hipLaunchKernelGGL((pcg_update_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, slot,
-                         max_iters, n_, Minv_.get(), p_[slot].get(), q_.get(), partA_.get(), x,
-                         r_.get(), z_.get(), partB_.get(), scal_.get());
+                         n_, Minv_.get(), p_[slot].get(), q_.get(), partA_.get(), x, r_.get(),
+                         z_.get(), partB_.get(), scal_.get());
     }
   });
   HIP_OK(hipStreamEndCapture(s, &g));
   HIP_OK(hipGraphInstantiate(&chunk_, g, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(g));
   chunk_x_ = x;
-  chunk_max_ = max_iters;
-  chunk_tol2_ = tol2;
 }
 
 PcgResult BlockPcg::solve(const double* b, double* x, int max_iters, double rel_tol,
@@ -391,10 +395,11 @@ PcgResult BlockPcg::solve(const double* b, double* x, int max_iters, double rel_
   SYMHIP_CHECK(max_iters >= 1, "BlockPcg: bad iteration limit");
   (void)check_every;
   const double tol2 = rel_tol * rel_tol;
-  capture_chunk(x, max_iters, tol2, s);
+  capture_chunk(x, s);
   dispatch_nd(nd_, [&]<int ND>() {
     hipLaunchKernelGGL((pcg_init_kernel<ND>), dim3(n_upd_blocks_), dim3(256), 0, s, n_, b,
-                       Minv_.get(), x, r_.get(), z_.get(), p_[1].get(), partB_.get(), scal_.get());
+                       Minv_.get(), x, r_.get(), z_.get(), p_[1].get(), partB_.get(), scal_.get(),
+                       tol2, max_iters);
   });
   HIP_OK(hipGetLastError());
   // chunks of kChunk iterations cover k = 0 .. max_iters (the last spmv only checks)

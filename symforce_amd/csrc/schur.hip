@@ -350,6 +350,7 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // (transposed). Z is read as TW (FP32 in the FP32 form) and widened: the sums are FP64 on the
 // matrix cores either way; the full storage is written as TS.
 constexpr int kRcsChunk = 512;  // pairs per work item at most
+constexpr int kRcsBatch = 8;    // pairs per batch (a multiple of 4: 3 K entries per pair, 4 per MFMA)
 constexpr bool kRcsHeaviestFirst = true;  // order each XCD's items by size (else by camera row)
 constexpr int kRcsFirst = 1, kRcsAtomic = 2;
 template <int ND, typename TW, typename TS>
@@ -380,22 +381,23 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   const bool live = row < ND;
   const int rowoff = (live ? row : 0) * 3;
   d4 acc0 = {}, acc1 = {}, acc2 = {};
-  // K entry 4h + kg of a batch belongs to its pair p = (4h + kg) / 3 (column (4h + kg) % 3), which
-  // is pair plo = 4h / 3 or plo + 1: a select between two wave-uniform indices. Indices past the
-  // item's end come from the next pairs or the padding (valid observations), masked out by ok.
-  // Enough waves stay resident (about 110 registers each) to overlap one wave's loads with the
-  // others' MFMAs.
-  constexpr int ZS = z_stride<ND, TW>();
-  for (int q = q_begin; q < q_end; q += 8) {
-    int xs[8], ys[8];
+  // A batch is kRcsBatch pairs = 3 kRcsBatch K entries = kRcsBatch * 3 / 4 MFMAs; its indices are
+  // wave-uniform scalar loads, its Z values 2 per MFMA and lane, all issued before the MFMAs (the
+  // loop is latency-bound: one round trip for the indices, one for Z, per batch). K entry 4h + kg
+  // belongs to pair p = (4h + kg) / 3 (column (4h + kg) % 3), which is pair plo = 4h / 3 or
+  // plo + 1: a select between two wave-uniform indices. Indices past the item's end come from the
+  // next pairs or the padding (valid observations), masked out by ok.
+  constexpr int ZS = z_stride<ND, TW>(), NB = kRcsBatch, NH = NB * 3 / 4;
+  for (int q = q_begin; q < q_end; q += NB) {
+    int xs[NB], ys[NB];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < NB; ++p) {
       xs[p] = pairs[2 * (q + p)];
       ys[p] = pairs[2 * (q + p) + 1];
     }
-    double av[6], bv[6];
+    double av[NH], bv[NH];
 #pragma unroll
-    for (int h = 0; h < 6; ++h) {
+    for (int h = 0; h < NH; ++h) {
       const int K = 4 * h + kg;
       const int p = K / 3, col = K - 3 * p;
       const int plo = (4 * h) / 3;
@@ -408,12 +410,15 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
       av[h] = ok ? za : 0.0;
       bv[h] = ok ? zb : 0.0;
     }
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[2], bv[2], acc2, 0, 0, 0);
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[3], bv[3], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[4], bv[4], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[5], bv[5], acc2, 0, 0, 0);
+    // all of the batch's loads go out before the first MFMA waits (left alone, the scheduler
+    // interleaves them with the MFMAs four at a time, a round trip per four loads)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int h = 0; h < NH; h += 3) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h + 1], bv[h + 1], acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h + 2], bv[h + 2], acc2, 0, 0, 0);
+    }
   }
   const d4 sum = acc0 + acc1 + acc2;
   const int j = row;
@@ -578,11 +583,14 @@ class SchurBackend final : public SystemBackend {
     out[6] = n_pairs_;
     out[7] = n_obs_;
   }
-  void begin_run() override { exact_now_ = false; }
+  void begin_run() override {
+    exact_now_ = false;
+    last_reduction_ = -1;
+  }
   void observe(const symhip_iteration_stats& rec) override {
-    if (use_pcg_ && exact_below_ > 0 && rec.update_accepted && rec.relative_reduction >= 0 &&
-        rec.relative_reduction < exact_below_)
-      exact_now_ = true;
+    if (!use_pcg_ || !rec.update_accepted || rec.relative_reduction < 0) return;
+    last_reduction_ = rec.relative_reduction;
+    if (exact_below_ > 0 && rec.relative_reduction < exact_below_) exact_now_ = true;
   }
   void log_systems(int max) override;
   int logged_systems() const override { return (int)logged_S_.size(); }
@@ -648,6 +656,7 @@ class SchurBackend final : public SystemBackend {
   // supernodal LDL^T when it does not reach the tolerance.
   bool use_pcg_ = false, pcg_strict_ = false, exact_now_ = false;
   double exact_below_ = 0;
+  double forcing_ = 0, last_reduction_ = -1;  // inexact-Newton forcing (pcg_forcing)
   int pcg_max_iters_ = 100;
   double pcg_rel_tol_ = 1e-6;
   BlockPcg pcg_;
@@ -875,8 +884,8 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   std::vector<int32_t> diag_blk(n_nodes_);
   for (int32_t a = 0; a < n_nodes_; ++a) diag_blk[a] = block_index(a, a);
   // Observation pairs per off-diagonal block (a > b; the diagonal blocks' sum over the camera's
-  // own observations is reduced per camera instead). The list is padded with 8 pairs of
-  // observation 0 so the assembly kernel's batches of 8 stay inside it.
+  // own observations is reduced per camera instead). The list is padded with kRcsBatch pairs of
+  // observation 0 so the assembly kernel's last batch stays inside it.
   std::vector<int32_t> pcount(n_blocks_ + 1, 0);
   for (int32_t p = 0; p < n_points_; ++p)
     for (int32_t x = pt_ptr[p]; x < pt_ptr[p + 1]; ++x)
@@ -886,7 +895,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
       }
   for (int32_t k = 0; k < n_blocks_; ++k) pcount[k + 1] += pcount[k];
   n_pairs_ = pcount[n_blocks_];
-  std::vector<int32_t> pairs(2 * ((size_t)pcount[n_blocks_] + 8), 0);
+  std::vector<int32_t> pairs(2 * ((size_t)pcount[n_blocks_] + kRcsBatch), 0);
   {
     std::vector<int32_t> fill(pcount.begin(), pcount.end() - 1);
     for (int32_t p = 0; p < n_points_; ++p)
@@ -993,6 +1002,8 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     pcg_strict_ = o.use_pcg == 2;
     SYMHIP_CHECK(o.pcg_exact_below_reduction >= 0, "pcg_exact_below_reduction must be >= 0");
     exact_below_ = o.pcg_exact_below_reduction;
+    SYMHIP_CHECK(o.pcg_forcing >= 0, "pcg_forcing must be >= 0");
+    forcing_ = o.pcg_forcing;
     pcg_max_iters_ = o.pcg_max_iters;
     pcg_rel_tol_ = o.pcg_rel_tol;
     // full symmetric block pattern: row a holds b for every lower block (a, b) and (b, a)
@@ -1256,7 +1267,16 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
         gnn_.preconditioner(pcg_.blocks(), pcg_.precond(), s);
       else
         pcg_.build_preconditioner(nullptr, s);
-      res = pcg_.solve(v_.get(), y_.get(), pcg_max_iters_, pcg_rel_tol_, s);
+      // inexact Newton: the tolerance follows the last accepted step's relative reduction
+      double tol = pcg_rel_tol_;
+      int max_iters = pcg_max_iters_;
+      if (forcing_ > 0 && last_reduction_ >= 0) {
+        tol = std::clamp(forcing_ * std::sqrt(last_reduction_), 1e-10, pcg_rel_tol_);
+        max_iters = (int)std::min(
+            20.0 * pcg_max_iters_,
+            std::ceil(pcg_max_iters_ * (1.0 + 2.0 * std::log10(pcg_rel_tol_ / tol))));
+      }
+      res = pcg_.solve(v_.get(), y_.get(), max_iters, tol, s);
     }
     ++pcg_solves_;
     pcg_iters_ += res.iterations;

@@ -77,6 +77,23 @@ def test_switch_to_exact_solves_reaches_direct_cost():
     assert best(fast) <= best(direct) * (1 + 1e-6)
 
 
+def test_forcing_tightens_pcg_and_reaches_direct_minimum():
+    """Inexact Newton: after an accepted step with relative reduction r the next PCG solve runs
+    to min(tol, c sqrt(r)) with a proportionally larger iteration limit. Alone it leaves LM's
+    early exit a few 1e-6 above the minimum (its last steps are still inexact); with a late
+    switch to the exact LDL^T it converges to the direct path's minimum."""
+    prob = capi.gen_ba_snavely(40, 3000, 13)
+    params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100,
+                                 early_exit_min_reduction=0.0)
+    direct, _ = run(prob, params)
+    forced, st = run(prob, params, use_pcg=1, pcg_max_iters=20, pcg_rel_tol=1e-2,
+                     pcg_forcing=0.3, pcg_exact_below_reduction=1e-5)
+    best = lambda r: min(it["new_error"] for it in r["iterations"])
+    assert st["fallbacks"] == 0 and 0 < st["solves"] < len(forced["iterations"]) - 1
+    assert st["max_iterations"] > 20  # the tightened solves ran past the base limit
+    assert abs(best(forced) - best(direct)) <= 1e-8 * best(direct)
+
+
 def test_pcg_rejected_on_cholesky_path():
     prob = capi.gen_pose_smoothing(10, 42)
     with pytest.raises(RuntimeError, match="Schur path only"):
