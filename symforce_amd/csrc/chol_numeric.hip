@@ -43,10 +43,12 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
 // factors the 4 x 4 diagonal block redundantly, forms w = c L4^-T for its rows and columns
 // (c = the published entries, L(i, k0 + u) = w_u / d_u) and applies
 // A(i, j) -= sum_u w_iu w_ju / d_u. The finished columns go to Lout (LDS; D on the diagonal),
-// written by the tx == 0 thread of each row. Nothing is masked: entries above the diagonal and
-// columns already eliminated pick up products of other such entries, but they are never read
-// again (the final values are in Lout), and the entries still to be eliminated only ever combine
-// published lower-triangle values.
+// written by the tx == 0 thread of each row. Register patches that are dead for the rest of the
+// sweep are skipped at compile time (kb is unrolled): columns y < kb (eliminated) and diagonal
+// rows x < kb (above the current block). Within the live patches nothing is masked: entries above
+// the diagonal and columns already eliminated pick up products of other such entries, but they
+// are never read again (the final values are in Lout), and the entries still to be eliminated
+// only ever combine published lower-triangle values.
 template <int W>
 __device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, double (&d)[4],
                                             double (&dinv)[4], double (&L4)[4][4]) {
@@ -56,7 +58,11 @@ __device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, doubl
 #pragma unroll
     for (int v = 0; v < u; ++v) dd -= L4[u][v] * L4[u][v] * d[v];
     d[u] = dd;
-    dinv[u] = 1.0 / dd;
+    // reciprocal by the hardware estimate and two Newton steps (the IEEE division sequence is
+    // twice as long, and these four sit on the sweep's critical path)
+    double r0 = __builtin_amdgcn_rcp(dd);
+    r0 = fma(r0, fma(-dd, r0, 1.0), r0);
+    dinv[u] = fma(r0, fma(-dd, r0, 1.0), r0);
 #pragma unroll
     for (int r = u + 1; r < 4; ++r) {
       double x = cb[u][k0 + r];
@@ -78,9 +84,11 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
 #pragma unroll 1
     for (int mm = 0; mm < 4; ++mm) {
       const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
+      auto row_live = [&](int x) { return x >= 4 || x >= kb; };
       if (uo >= 0 && uo < 4) {
 #pragma unroll
-        for (int x = 0; x < RX; ++x) cb[buf][uo][ty + 16 * x] = a[x][kb];
+        for (int x = 0; x < RX; ++x)
+          if (row_live(x)) cb[buf][uo][ty + 16 * x] = a[x][kb];
       }
       __syncthreads();
       double d[4], dinv[4], L4[4][4];
@@ -88,6 +96,7 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
       double wr[RX][4], wc[4][4];
 #pragma unroll
       for (int x = 0; x < RX; ++x) {
+        if (!row_live(x)) continue;
         const int i = ty + 16 * x;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -99,6 +108,7 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
       }
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
+        if (y < kb) continue;
         const int j = tx + 16 * y;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -111,14 +121,17 @@ __device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
       // wr <- L(i, k0 + u) = w_iu / d_u
 #pragma unroll
       for (int x = 0; x < RX; ++x)
+        if (row_live(x))
 #pragma unroll
-        for (int u = 0; u < 4; ++u) wr[x][u] *= dinv[u];
+          for (int u = 0; u < 4; ++u) wr[x][u] *= dinv[u];
 #pragma unroll
       for (int x = 0; x < RX; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y)
+        for (int y = 0; y < 4; ++y) {
+          if (!row_live(x) || y < kb) continue;
 #pragma unroll
           for (int u = 0; u < 4; ++u) a[x][y] -= wr[x][u] * wc[y][u];
+        }
       if (tx == 0) {
         // rows below the 4 x 4 block (row groups x > kb, and x == kb past it); the block itself
         // (rows k0 .. k0 + 3, all in group kb) gets D and L4
@@ -230,7 +243,6 @@ __device__ __forceinline__ void update_body(int s, int ti, int tj,
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const double* P = panel + sd.panel_off;
   const double* Ds = D + sd.col_begin;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   // stage both row tiles (all k at once: the loads are independent)
   // (unconditional loads from clamped addresses; an update item exists only when mb > 0)
 #pragma unroll
@@ -244,43 +256,47 @@ __device__ __forceinline__ void update_body(int s, int ti, int tj,
     As[k][r] = (k < n && ri < mb) ? va : 0.0;
     Bs[k][r] = (k < n && rj < mb) ? vb : 0.0;
   }
-  // scatter targets of this thread's 16 entries, fetched while the tiles load and before the
-  // product loop (the row positions depend on the metadata: two dependent loads, off the
-  // critical path here). Indices are clamped into the column's update segment (rows
-  // j .. mb - 1); clamped entries are masked at the scatter.
+  // The product on the FP64 matrix cores: wave w computes rows 16w .. 16w + 15 of the tile as
+  // four 16 x 16 output blocks c (columns 16c ..), K = 64 in 16 steps of v_mfma_f64_16x16x4f64.
+  // Lane l feeds A(i = l % 16, k = l / 16) = As[k][16w + i] and B(k, j = l % 16) = Bs[k][16c + j]
+  // and holds D(l / 16 + 4 r, l % 16) of block c in accumulator register r: 5 LDS reads per 4
+  // MFMAs (the FMA form needed 8 per 16 FMAs and was bound by them).
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // scatter targets of this lane's 16 entries (rows 16w + lk + 4r, columns 16c + lr of the tile),
+  // fetched while the tiles load and before the product loop (the row positions depend on the
+  // metadata: two dependent loads, off the critical path here). Indices are clamped into the
+  // column's update segment (rows j .. mb - 1); clamped entries are masked at the scatter.
   int64_t cbase[4];
   int32_t rel[4][4];
 #pragma unroll
-  for (int y = 0; y < 4; ++y) {
-    const int j = min(tj * kTile + tx + 16 * y, mb - 1);
-    cbase[y] = scat_col[sd.rows_off + n + j];
+  for (int c = 0; c < 4; ++c) {
+    const int j = min(tj * kTile + 16 * c + lr, mb - 1);
+    cbase[c] = scat_col[sd.rows_off + n + j];
     const int32_t rbase = scat_rel[sd.rows_off + n + j];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
-      rel[y][x] = upd_rel[rbase + min(max(ti * kTile + ty + 16 * x, j), mb - 1)];
+    for (int r = 0; r < 4; ++r)
+      rel[c][r] = upd_rel[rbase + min(max(ti * kTile + 16 * wave + lk + 4 * r, j), mb - 1)];
   }
   __syncthreads();
-  double acc[4][4] = {};
-#pragma unroll 16
-  for (int k = 0; k < kMaxCols; ++k) {
-    double av[4], bv[4];
+  d4 acc[4] = {};
+#pragma unroll 4
+  for (int k0 = 0; k0 < kMaxCols; k0 += 4) {
+    const double av = As[k0 + lk][16 * wave + lr];
+    double bv[4];
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      av[x] = As[k][ty + 16 * x];
-      bv[x] = Bs[k][tx + 16 * x];
-    }
+    for (int c = 0; c < 4; ++c) bv[c] = Bs[k0 + lk][16 * c + lr];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y) acc[x][y] += av[x] * bv[y];
+    for (int c = 0; c < 4; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
   }
 #pragma unroll
-  for (int y = 0; y < 4; ++y) {
-    const int j = tj * kTile + tx + 16 * y;
+  for (int c = 0; c < 4; ++c) {
+    const int j = tj * kTile + 16 * c + lr;
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int i = ti * kTile + ty + 16 * x;
-      if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[y] + rel[y][x], -acc[x][y]);
+    for (int r = 0; r < 4; ++r) {
+      const int i = ti * kTile + 16 * wave + lk + 4 * r;
+      if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[c] + rel[c][r], -acc[c][r]);
     }
   }
 }
@@ -409,6 +425,12 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ it
   const double* P = panel + sd.panel_off + n;
   const int32_t* rw = rows + sd.rows_off + n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Every block stages L11 and reads y_own up front: only the last block of s uses them, but this
+  // way their loads overlap the partial sums instead of following the arrival counter (y_own is
+  // written by no other block of this launch).
+  stage_l11(l11 + sd.l11_off, n, Ls);
+  double yown = 0;
+  if (threadIdx.x < n) yown = yout[sd.col_begin + threadIdx.x];
   if (mb > 0) {
     constexpr int kPer = kMaxCols / 4;
     double acc[kPer] = {};
@@ -441,14 +463,12 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ it
   __syncthreads();
   if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
   __threadfence();
-  stage_l11(l11 + sd.l11_off, n, Ls);
-  __syncthreads();
   if (threadIdx.x < 64) {
     const int r = threadIdx.x;
     double yi = 0;
     if (r < n)
-      yi = yout[sd.col_begin + r] - __hip_atomic_load(&part[sd.col_begin + r], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
+      yi = yown - __hip_atomic_load(&part[sd.col_begin + r], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int k = kMaxCols - 1; k >= 0; --k) yi -= Ls[k][r] * lane_value(yi, k);
     if (r < n) yout[sd.col_begin + r] = yi;
