@@ -262,22 +262,27 @@ __device__ __forceinline__ void update_body(int s, int ti, int tj,
   // and holds D(l / 16 + 4 r, l % 16) of block c in accumulator register r: 5 LDS reads per 4
   // MFMAs (the FMA form needed 8 per 16 FMAs and was bound by them).
   typedef double d4 __attribute__((ext_vector_type(4)));
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar metadata
   const int lr = lane & 15, lk = lane >> 4;
-  // scatter targets of this lane's 16 entries (rows 16w + lk + 4r, columns 16c + lr of the tile),
-  // fetched while the tiles load and before the product loop (the row positions depend on the
-  // metadata: two dependent loads, off the critical path here). Indices are clamped into the
-  // column's update segment (rows j .. mb - 1); clamped entries are masked at the scatter.
-  int64_t cbase[4];
-  int32_t rel[4][4];
+  // Scatter targets, fetched while the tiles load and before the product loop (the row
+  // positions depend on the metadata: two dependent loads, off the critical path here). The
+  // scatter runs column by column: wave w takes tile columns w, w + 4, ..., w + 60 and lane l row
+  // l, so one atomic instruction covers 64 consecutive rows of one target column (a few cache
+  // lines) instead of 16 columns x 4 rows. Row indices are clamped into the column's update
+  // segment (rows j .. mb - 1); clamped entries are masked at the scatter.
+  constexpr int kCols = kTile / 4;
+  int64_t cbase[kCols];
+  int32_t rel[kCols];
+  {
+    const int i = ti * kTile + lane;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = min(tj * kTile + 16 * c + lr, mb - 1);
-    cbase[c] = scat_col[sd.rows_off + n + j];
-    const int32_t rbase = scat_rel[sd.rows_off + n + j];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      rel[c][r] = upd_rel[rbase + min(max(ti * kTile + 16 * wave + lk + 4 * r, j), mb - 1)];
+    for (int q = 0; q < kCols; ++q) {
+      const int j = min(tj * kTile + wave + 4 * q, mb - 1);
+      cbase[q] = scat_col[sd.rows_off + n + j];
+      const int32_t rbase = scat_rel[sd.rows_off + n + j];
+      rel[q] = upd_rel[rbase + min(max(i, j), mb - 1)];
+    }
   }
   __syncthreads();
   d4 acc[4] = {};
@@ -290,14 +295,20 @@ __device__ __forceinline__ void update_body(int s, int ti, int tj,
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
   }
+  // transpose through LDS (the A tile's space): T[column][row]
+  __syncthreads();
+  auto T = As;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = tj * kTile + 16 * c + lr;
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = ti * kTile + 16 * wave + lk + 4 * r;
-      if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[c] + rel[c][r], -acc[c][r]);
-    }
+    for (int r = 0; r < 4; ++r) T[16 * c + lr][16 * wave + lk + 4 * r] = acc[c][r];
+  __syncthreads();
+  const int i = ti * kTile + lane;
+#pragma unroll
+  for (int q = 0; q < kCols; ++q) {
+    const int jl = wave + 4 * q, j = tj * kTile + jl;
+    const double v = T[jl][lane];
+    if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[q] + rel[q], -v);
   }
 }
 
