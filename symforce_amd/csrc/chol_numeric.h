@@ -17,13 +17,17 @@ class SupernodalLDLT {
   void setup(const CholSymbolic& sym, const int32_t* colptr, const int32_t* rowind,
              hipStream_t s);
   // Assembles A (device values in the CSC order given to setup) into panels and factorizes.
-  void factorize(const double* a_values, hipStream_t s);
-  // Panels already assembled by the caller (e.g. the Schur path writes them directly).
-  // The per-level launch sequence is captured in a hipGraph on the first call and replayed.
-  void factorize_assembled(hipStream_t s);
+  // With b (device, original order) the forward solve L y = P b runs inside the factorization's
+  // diagonal kernels, and solve_fused finishes x = A^-1 b.
+  void factorize(const double* a_values, hipStream_t s, const double* b = nullptr);
+  // Panels already assembled by the caller (e.g. the Schur path writes them directly). The
+  // per-level launch sequence is captured in a hipGraph per b on the first call and replayed.
+  void factorize_assembled(hipStream_t s, const double* b = nullptr);
   // x = A^-1 b for device vectors in the original (unpermuted) order; x may alias b.
   // Captured per (b, x) pair and replayed.
   void solve(const double* b, double* x, hipStream_t s);
+  // x = A^-1 b for the b given to the last factorization (backward half only); x may alias b.
+  void solve_fused(double* x, hipStream_t s);
   SupernodalLDLT() = default;
   SupernodalLDLT(const SupernodalLDLT&) = delete;
   SupernodalLDLT& operator=(const SupernodalLDLT&) = delete;
@@ -36,15 +40,20 @@ class SupernodalLDLT {
   int64_t assembly_nnz() const { return (int64_t)asm_map_.size(); }
 
  private:
-  void launch_factorize(hipStream_t s);
+  void launch_factorize(const double* b, hipStream_t s);
   void launch_solve(const double* b, double* x, hipStream_t s);
-  hipGraphExec_t fact_graph_ = nullptr;
-  struct SolveGraph {
+  void launch_backward(double* x, hipStream_t s);
+  struct CachedGraph {
     const double* b;
     double* x;
     hipGraphExec_t exec;
   };
-  std::vector<SolveGraph> solve_graphs_;  // one per (b, x) pair seen (a few state blocks)
+  template <typename F>
+  hipGraphExec_t cached_graph(std::vector<CachedGraph>& cache, const double* b, double* x,
+                              hipStream_t s, F&& launches);
+  // captured launch sequences, one per argument pair seen (a few state blocks)
+  std::vector<CachedGraph> fact_graphs_, solve_graphs_, bwd_graphs_;
+  bool fused_ = false;  // the last factorization ran the forward solve
   CholSymbolic sym_;
   DBuf<double> panel_, l11_, D_, y_, yout_, part_;
   DBuf<int32_t> counter_;  // per-supernode arrival counters of the backward pass (self-resetting)

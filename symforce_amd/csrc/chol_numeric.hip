@@ -34,196 +34,232 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
     panel[map[q]] = vals[q];
 }
 
-// Right-looking blocked LDL^T of a 64-column panel held as 4 x 4 register patches: rows
-// 0..63 are the diagonal block (padded with identity), rows 64..16 RX - 1 are below rows that are
-// eliminated in the same sweep (their final values are L21 = A21 L11^-T D^-1). Thread (ty, tx)
-// owns rows ty + 16x (x < RX) and cols tx + 16y (y < 4). Step m eliminates columns
-// k0 = 4m .. k0 + 3 (register column kb = k0 / 16, owner threads tx = k0 % 16 + u): the owners
-// publish the four columns through LDS (one barrier per step, double-buffered), every thread
-// factors the 4 x 4 diagonal block redundantly, forms w = c L4^-T for its rows and columns
-// (c = the published entries, L(i, k0 + u) = w_u / d_u) and applies
-// A(i, j) -= sum_u w_iu w_ju / d_u. The finished columns go to Lout (LDS; D on the diagonal),
-// written by the tx == 0 thread of each row. Register patches that are dead for the rest of the
-// sweep are skipped at compile time (kb is unrolled): columns y < kb (eliminated) and diagonal
-// rows x < kb (above the current block). Within the live patches nothing is masked: entries above
-// the diagonal and columns already eliminated pick up products of other such entries, but they
-// are never read again (the final values are in Lout), and the entries still to be eliminated
-// only ever combine published lower-triangle values.
-template <int W>
-__device__ __forceinline__ void ldlt4_block(const double (*cb)[W], int k0, double (&d)[4],
-                                            double (&dinv)[4], double (&L4)[4][4]) {
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    double dd = cb[u][k0 + u];
-#pragma unroll
-    for (int v = 0; v < u; ++v) dd -= L4[u][v] * L4[u][v] * d[v];
-    d[u] = dd;
-    // reciprocal by the hardware estimate and two Newton steps (the IEEE division sequence is
-    // twice as long, and these four sit on the sweep's critical path)
-    double r0 = __builtin_amdgcn_rcp(dd);
-    r0 = fma(r0, fma(-dd, r0, 1.0), r0);
-    dinv[u] = fma(r0, fma(-dd, r0, 1.0), r0);
-#pragma unroll
-    for (int r = u + 1; r < 4; ++r) {
-      double x = cb[u][k0 + r];
-#pragma unroll
-      for (int v = 0; v < u; ++v) x -= L4[r][v] * L4[u][v] * d[v];
-      L4[r][u] = x * dinv[u];
-    }
-  }
-}
-
-constexpr int kLoutLd = kMaxCols + 2;  // Lout row stride (doubles): 16-byte rows, fewer conflicts
-
-template <int RX>
-__device__ __forceinline__ void ldlt_panel_registers(double (&a)[RX][4],
-                                                     double (*cb)[4][16 * RX],
-                                                     double (*Lout)[kLoutLd], int ty, int tx) {
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll 1
-    for (int mm = 0; mm < 4; ++mm) {
-      const int k0 = 16 * kb + 4 * mm, buf = mm & 1, uo = tx - 4 * mm;
-      auto row_live = [&](int x) { return x >= 4 || x >= kb; };
-      if (uo >= 0 && uo < 4) {
-#pragma unroll
-        for (int x = 0; x < RX; ++x)
-          if (row_live(x)) cb[buf][uo][ty + 16 * x] = a[x][kb];
-      }
-      __syncthreads();
-      double d[4], dinv[4], L4[4][4];
-      ldlt4_block<16 * RX>(cb[buf], k0, d, dinv, L4);
-      double wr[RX][4], wc[4][4];
-#pragma unroll
-      for (int x = 0; x < RX; ++x) {
-        if (!row_live(x)) continue;
-        const int i = ty + 16 * x;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double c = cb[buf][u][i];
-#pragma unroll
-          for (int v = 0; v < u; ++v) c -= wr[x][v] * L4[u][v];
-          wr[x][u] = c;
-        }
-      }
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        if (y < kb) continue;
-        const int j = tx + 16 * y;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          double c = cb[buf][u][j];
-#pragma unroll
-          for (int v = 0; v < u; ++v) c -= wc[y][v] * L4[u][v];
-          wc[y][u] = c;
-        }
-      }
-      // wr <- L(i, k0 + u) = w_iu / d_u
-#pragma unroll
-      for (int x = 0; x < RX; ++x)
-        if (row_live(x))
-#pragma unroll
-          for (int u = 0; u < 4; ++u) wr[x][u] *= dinv[u];
-#pragma unroll
-      for (int x = 0; x < RX; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          if (!row_live(x) || y < kb) continue;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) a[x][y] -= wr[x][u] * wc[y][u];
-        }
-      if (tx == 0) {
-        // rows below the 4 x 4 block (row groups x > kb, and x == kb past it); the block itself
-        // (rows k0 .. k0 + 3, all in group kb) gets D and L4
-#pragma unroll
-        for (int x = kb; x < RX; ++x) {
-          const int i = ty + 16 * x;
-          if (x == kb && i < k0 + 4) {
-            if (i >= k0) {
-              const int r = i - k0;
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                double v = 0.0;
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr)
-                  if (rr == r) v = u < rr ? L4[rr][u] : (u == rr ? d[u] : 0.0);
-                Lout[i][k0 + u] = v;
-              }
-            }
-          } else {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) Lout[i][k0 + u] = wr[x][u];
-          }
-        }
-      }
-    }
-  }
-}
-
-// LDS (doubles) of the two factorization kernels.
-constexpr int kDiagRX = 8;  // row groups of the diagonal task: 4 of the diagonal block + 4 below
-constexpr int kDiagSmem = 2 * 4 * 16 * kDiagRX + 16 * kDiagRX * kLoutLd;
+// Diagonal task (s, chunk): one 512-thread block eliminates the 64 x 64 diagonal block of
+// supernode s together with the chunk's 64 below rows in one right-looking sweep of 8-column
+// steps (the diagonal block is factored redundantly by every chunk of s; chunk 0 writes L11 to
+// l11 and D), leaving L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
+//
+// The 128 x 64 working matrix (rows 0..63 the diagonal block padded with identity, rows 64..127
+// the chunk's below rows) lives in matrix-core accumulators: wave w holds rows 16w .. 16w + 15 as
+// four 16 x 16 tiles. Step k0 = 0, 8, .., 56:
+//  1. the lanes holding columns k0 .. k0 + 7 publish them to cb (LDS);
+//  2. row owners (thread t < 128 owns row t) factor the 8 x 8 pivot block (redundantly, a chain
+//     of 8 reciprocals), form their row's w = c L8^-T (c = the row's published entries) and
+//     L = w D^-1, and write w to Wt and L to Lc (the pivot rows get L8 and D instead);
+//  3. every wave applies A(i, j) -= sum_u L(i, u) w(j, u) to its live tiles with
+//     v_mfma_f64_16x16x4f64 (operands read straight from Lc and Wt).
+// Each row's elimination is computed once, and an update needs 10 LDS reads per lane per step
+// (a register-patch FMA form read 64, and was bound by them). Tiles that are dead for the rest of
+// the sweep are skipped (the step loop is unrolled; the row test is wave-uniform): columns below
+// k0 + 8 and diagonal-block row tiles above it. Within live tiles nothing is masked: entries
+// above the diagonal and finished columns pick up products of other such entries but are never
+// read again (the final values are in Lc), and the entries still to be eliminated only ever
+// combine lower-triangle values.
+constexpr int kDiagThreads = 512;
+constexpr int kStep = 8;                   // columns eliminated per sweep step
+constexpr int kDiagRows = 2 * kMaxCols;    // diagonal block + one chunk of below rows
+// LDS layouts, padded so the owners' row-wise writes, the publishing lanes' writes and the
+// matrix-core operand reads are free of bank conflicts (row-major arrays with 8-double rows cost
+// 16-way conflicts on the owners' writes, four fifths of the owner phase)
+constexpr int kCbLd = kDiagRows + 4;       // cb[u][row]: published columns
+constexpr int kLcLd = kDiagRows + 2;       // Lc[col][row]: the working matrix, then L and D
+constexpr int kDiagSmem = kStep * kCbLd + kStep * kDiagRows + kMaxCols * kLcLd;
 constexpr int kUpdSmem = 2 * kMaxCols * (kTile + 1);
 
-// Diagonal task (s, chunk), one 256-thread block: eliminates the 64 x 64 diagonal block of
-// supernode s together with the chunk's 64 below rows in one right-looking sweep (the diagonal
-// block is factored redundantly by every chunk of s; chunk 0 writes L11 to l11 and D), leaving
-// L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
+// 1 / d by the hardware estimate and two Newton steps (the IEEE division sequence is twice as
+// long, and these sit on the sweep's critical path).
+__device__ __forceinline__ double refined_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// Step 2 for row t: cb[u][i] = A(i, k0 + u) after the previous steps; writes Wt[u][t] = w_u and
+// Lc[k0 + u][t] = L(t, k0 + u).
+__device__ __forceinline__ void sweep_owner(int k0, int t, const double (*cb)[kCbLd],
+                                            double (*Wt)[kDiagRows], double (*Lc)[kLcLd]) {
+  double d[kStep], dinv[kStep], L8[kStep][kStep];
+  // pivot block, column by column: with wp_uv = L_uv d_v (row u's, formed when column u starts),
+  // d_u = A_uu - sum_v wp_uv L_uv and L_ru = (A_ru - sum_v L_rv wp_uv) / d_u
+#pragma unroll
+  for (int u = 0; u < kStep; ++u) {
+    double wp[kStep];
+#pragma unroll
+    for (int v = 0; v < u; ++v) wp[v] = L8[u][v] * d[v];
+    double dd = cb[u][k0 + u];
+#pragma unroll
+    for (int v = 0; v < u; ++v) dd -= wp[v] * L8[u][v];
+    d[u] = dd;
+    dinv[u] = refined_rcp(dd);
+#pragma unroll
+    for (int r = u + 1; r < kStep; ++r) {
+      double x = cb[u][k0 + r];
+#pragma unroll
+      for (int v = 0; v < u; ++v) x -= L8[r][v] * wp[v];
+      L8[r][u] = x * dinv[u];
+    }
+  }
+  double w[kStep], l[kStep];
+#pragma unroll
+  for (int u = 0; u < kStep; ++u) {
+    double c = cb[u][t];
+#pragma unroll
+    for (int v = 0; v < u; ++v) c -= w[v] * L8[u][v];
+    w[u] = c;
+    l[u] = c * dinv[u];
+  }
+  // pivot row k0 + r: L8 row left of the diagonal, D on it (w_r = d_r), zero right of it
+  const int r = t - k0;
+#pragma unroll
+  for (int u = 0; u < kStep; ++u) {
+    Wt[u][t] = w[u];
+    Lc[k0 + u][t] = (r < 0 || r >= kStep || u < r) ? l[u] : (u == r ? w[u] : 0.0);
+  }
+}
+
+// Value of v in lane k (k wave-uniform), through two scalar lane reads: a short dependency for
+// the triangular recurrences (an LDS-based shuffle costs a full LDS round trip per step).
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int2 u = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(u.x, k),
+                                              __builtin_amdgcn_readlane(u.y, k)));
+}
+
+// Forward solve fused into the diagonal task (kFwd): with y = P b plus the updates of every
+// descendant (earlier levels), the block forms y_own = L11^-1 y_own (wave 0, lane r holding y_r)
+// once L11 is in LDS, chunk 0 writes it to yout, and the chunk's below rows get
+// y(row) -= L21(row, :) y_own atomically (siblings share ancestors). The same work as a separate
+// forward launch per level, without the launch.
+template <bool kFwd>
 __device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __restrict__ desc,
                                           double* __restrict__ panel, double* __restrict__ l11,
-                                          double* __restrict__ D, double* smem) {
-  constexpr int RX = kDiagRX;
-  auto cb = reinterpret_cast<double (*)[4][16 * RX]>(smem);
-  auto Lout = reinterpret_cast<double (*)[kLoutLd]>(smem + 2 * 4 * 16 * RX);
+                                          double* __restrict__ D, const int32_t* __restrict__ rows,
+                                          double* __restrict__ y, double* __restrict__ yout,
+                                          double* smem) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  auto cb = reinterpret_cast<double (*)[kCbLd]>(smem);
+  auto Wt = reinterpret_cast<double (*)[kDiagRows]>(smem + kStep * kCbLd);
+  auto Lc = reinterpret_cast<double (*)[kLcLd]>(smem + kStep * kCbLd + kStep * kDiagRows);
   const SnDesc sd = desc[s];
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   double* P = panel + sd.panel_off;
-  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-  double a[RX][4];
-  // Unconditional loads from clamped (valid) addresses, then selects: a guarded load would be
-  // compiled as a branch and wait per element.
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  // forward-solve operands, fetched up front (clamped indices, selected at use)
+  double y_in = 0.0;
+  int32_t y_row = 0;
+  if (kFwd && wave == 0) {
+    y_in = y[sd.col_begin + min(lane, n - 1)];
+    y_row = rows[sd.rows_off + (mb > 0 ? n + min(chunk * kMaxCols + lane, mb - 1) : 0)];
+  }
+  // Stage the working matrix in Lc with coalesced loads (consecutive threads take consecutive
+  // rows of one column): one unconditional load per element from a clamped valid address, then
+  // a select (a guarded load is compiled as a branch that waits for each load in turn).
+  constexpr int kPer = kDiagRows * kMaxCols / kDiagThreads;
+  double v[kPer];
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int i = ty + 16 * x, j = tx + 16 * y;
-      const int ic = min(i, n - 1), jc = min(j, n - 1);
-      const double va = P[(int64_t)jc * ld + ic];
-      const int r = chunk * 64 + i;
-      const double vw = P[(int64_t)jc * ld + (mb > 0 ? n + min(r, mb - 1) : ic)];
-      a[x][y] = (i < n && j < n && i >= j) ? va : (i == j ? 1.0 : 0.0);
-      a[x + 4][y] = (r < mb && j < n) ? vw : 0.0;
-    }
-  ldlt_panel_registers<RX>(a, cb, Lout, ty, tx);
-  __syncthreads();
-  if (chunk == 0) {
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int i = ty + 16 * x, j = tx + 16 * y;
-        const double v = Lout[i][j];
-        if (i < n && j < n && i > j) l11[sd.l11_off + (int64_t)j * n + i] = v;
-        if (i == j && i < n) D[sd.col_begin + i] = v;
-      }
+  for (int q = 0; q < kPer; ++q) {  // all loads first
+    const int idx = tid + kDiagThreads * q;
+    const int i = idx & (kDiagRows - 1), j = idx / kDiagRows;
+    const int jc = min(j, n - 1), ic = min(i, n - 1);
+    const int r = chunk * kMaxCols + i - kMaxCols;
+    const int row = (i >= kMaxCols) & (mb > 0) ? n + min(r, mb - 1) : ic;
+    v[q] = P[(int64_t)jc * ld + row];
   }
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + kDiagThreads * q;
+    const int i = idx & (kDiagRows - 1), j = idx / kDiagRows;
+    const int r = chunk * kMaxCols + i - kMaxCols;
+    const int below = i >= kMaxCols;
+    // below rows: inside the chunk; diagonal block: lower triangle inside n x n, identity padding
+    const int keep = (below & (r < mb) & (j < n)) | (!below & (i < n) & (j < n) & (i >= j));
+    const double fill = (!below & (i == j)) ? 1.0 : 0.0;
+    Lc[j][i] = keep ? v[q] : fill;
+  }
+  __syncthreads();
+  // Wave w holds rows 16w .. 16w + 15 as four 16 x 16 accumulator tiles (columns 16c ..), in the
+  // v_mfma_f64_16x16x4f64 layout: acc[c][r] = A(16w + lane / 16 + 4r, 16c + lane % 16).
+  d4 acc[4];
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int r = chunk * 64 + ty + 16 * x, j = tx + 16 * y;
-      const double v = Lout[64 + ty + 16 * x][j];
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[c][r] = Lc[16 * c + lr][16 * wave + lk + 4 * r];
+  // (the first barrier of the sweep orders these reads before the owners' Lc writes)
+  const bool below_wave = wave >= kMaxCols / 16;
+#pragma unroll
+  for (int st = 0; st < kMaxCols / kStep; ++st) {
+    const int k0 = kStep * st;
+    {
+      // publish columns k0 .. k0 + 7 (tile column k0 / 16, lanes k0 % 16 ..) of the live rows
+      const int off = k0 & 15, cp = k0 >> 4;
+      if ((below_wave || 16 * wave + 15 >= k0) && lr >= off && lr < off + kStep) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cb[lr - off][16 * wave + lk + 4 * r] = acc[cp][r];
+      }
+    }
+    __syncthreads();
+    if (tid < kDiagRows) sweep_owner(k0, tid, cb, Wt, Lc);
+    __syncthreads();
+    const int kn = k0 + kStep;  // first column still to be eliminated
+    if (kn < kMaxCols && (below_wave || 16 * wave + 15 >= kn)) {
+      // A(i, j) -= sum_u L(i, k0 + u) w(j, u) on the matrix cores, two K = 4 halves per tile:
+      // lane l feeds A(l % 16, l / 16) = -L(16w + l % 16, k0 + 4h + l / 16) and
+      // B(l / 16, l % 16) = w(16c + l % 16, 4h + l / 16)
+      const double a0 = -Lc[k0 + lk][16 * wave + lr];
+      const double a1 = -Lc[k0 + 4 + lk][16 * wave + lr];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (16 * c + 15 < kn) continue;  // tile columns already eliminated
+        const double b0 = Wt[lk][16 * c + lr], b1 = Wt[4 + lk][16 * c + lr];
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[c], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();
+  if (kFwd && wave == 0) {
+    const int r = lane;
+    double yi = r < n ? y_in : 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) yi -= (r > k ? Lc[k][r] : 0.0) * lane_value(yi, k);
+    if (chunk == 0 && r < n) yout[sd.col_begin + r] = yi;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) acc += Lc[k][kMaxCols + r] * lane_value(yi, k);
+    if (chunk * kMaxCols + r < mb) unsafeAtomicAdd(&y[y_row], -acc);
+  }
+  // write back with coalesced stores: L11 (strict lower, column-major n x n) and D from chunk 0,
+  // the chunk's L21 rows into the panel (the thread index is passed through an empty asm so the
+  // compiler recomputes these addresses here instead of holding the load phase's across the sweep)
+  int tid_w = tid;
+  asm volatile("" : "+v"(tid_w));
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid_w + kDiagThreads * q;
+    const int i = idx & (kDiagRows - 1), j = idx / kDiagRows;
+    const double v = Lc[j][i];
+    if (i < kMaxCols) {
+      if (chunk == 0 && i < n && j < n) {
+        if (i > j) l11[sd.l11_off + (int64_t)j * n + i] = v;
+        if (i == j) D[sd.col_begin + i] = v;
+      }
+    } else {
+      const int r = chunk * kMaxCols + i - kMaxCols;
       if (r < mb && j < n) P[(int64_t)j * ld + n + r] = v;
     }
+  }
 }
 
-__global__ void __launch_bounds__(256) diag_trsm_kernel(const int32_t* __restrict__ items,
-                                                        const SnDesc* __restrict__ desc,
-                                                        double* __restrict__ panel,
-                                                        double* __restrict__ l11,
-                                                        double* __restrict__ D) {
+template <bool kFwd>
+__global__ void __launch_bounds__(kDiagThreads) diag_trsm_kernel(
+    const int32_t* __restrict__ items, const SnDesc* __restrict__ desc, double* __restrict__ panel,
+    double* __restrict__ l11, double* __restrict__ D, const int32_t* __restrict__ rows,
+    double* __restrict__ y, double* __restrict__ yout) {
   __shared__ double smem[kDiagSmem];
-  diag_body(items[2 * blockIdx.x], items[2 * blockIdx.x + 1], desc, panel, l11, D, smem);
+  diag_body<kFwd>(items[2 * blockIdx.x], items[2 * blockIdx.x + 1], desc, panel, l11, D, rows, y,
+                  yout, smem);
 }
 
 // Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj), scattered as -U
@@ -354,14 +390,6 @@ __device__ __forceinline__ void stage_l11(const double* Lc, int n, double (*Ls)[
     const double v = Lc[(int64_t)min(k, n - 1) * n + min(i, n - 1)];  // clamped, then masked
     Ls[i][k] = (i < n && k < n && i > k) ? v : 0.0;
   }
-}
-
-// Value of v in lane k (k wave-uniform), through two scalar lane reads: a short dependency for
-// the triangular recurrences (an LDS-based shuffle costs a full LDS round trip per step).
-__device__ __forceinline__ double lane_value(double v, int k) {
-  const int2 u = __builtin_bit_cast(int2, v);
-  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(u.x, k),
-                                              __builtin_amdgcn_readlane(u.y, k)));
 }
 
 constexpr int kSolveRows = 256;  // below-rows per solve block (one per thread in the forward pass)
@@ -582,7 +610,7 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   HIP_OK(hipStreamSynchronize(s));
 }
 
-void SupernodalLDLT::factorize(const double* a_values, hipStream_t s) {
+void SupernodalLDLT::factorize(const double* a_values, hipStream_t s, const double* b) {
   fill_zero(panel_.get(), sym_.panel_size(), s);
   const int64_t nnz = (int64_t)asm_map_.size();
   if (nnz) {
@@ -590,12 +618,12 @@ void SupernodalLDLT::factorize(const double* a_values, hipStream_t s) {
     hipLaunchKernelGGL(assemble_kernel, dim3(grid), dim3(256), 0, s, nnz, asm_map_.get(), a_values,
                        panel_.get());
   }
-  factorize_assembled(s);
+  factorize_assembled(s, b);
 }
 
 SupernodalLDLT::~SupernodalLDLT() {
-  if (fact_graph_) (void)hipGraphExecDestroy(fact_graph_);
-  for (auto& g : solve_graphs_) (void)hipGraphExecDestroy(g.exec);
+  for (auto* cache : {&fact_graphs_, &solve_graphs_, &bwd_graphs_})
+    for (auto& g : *cache) (void)hipGraphExecDestroy(g.exec);
 }
 
 namespace {
@@ -612,35 +640,55 @@ hipGraphExec_t capture(hipStream_t s, F&& launches) {
 }
 }  // namespace
 
-void SupernodalLDLT::factorize_assembled(hipStream_t s) {
+void SupernodalLDLT::factorize_assembled(hipStream_t s, const double* b) {
   if (sym_.n_levels == 0) return;
-  if (!fact_graph_) fact_graph_ = capture(s, [&] { launch_factorize(s); });
-  HIP_OK(hipGraphLaunch(fact_graph_, s));
+  fused_ = b != nullptr;
+  HIP_OK(hipGraphLaunch(cached_graph(fact_graphs_, b, nullptr, s,
+                                     [&] { launch_factorize(b, s); }), s));
 }
 
 void SupernodalLDLT::solve(const double* b, double* x, hipStream_t s) {
   if (sym_.n == 0) return;
-  for (const auto& g : solve_graphs_)
-    if (g.b == b && g.x == x) {
-      HIP_OK(hipGraphLaunch(g.exec, s));
-      return;
-    }
-  if (solve_graphs_.size() >= 8) {  // bound the cache; the oldest goes
-    (void)hipGraphExecDestroy(solve_graphs_.front().exec);
-    solve_graphs_.erase(solve_graphs_.begin());
-  }
-  solve_graphs_.push_back({b, x, capture(s, [&] { launch_solve(b, x, s); })});
-  HIP_OK(hipGraphLaunch(solve_graphs_.back().exec, s));
+  HIP_OK(hipGraphLaunch(cached_graph(solve_graphs_, b, x, s, [&] { launch_solve(b, x, s); }), s));
 }
 
-void SupernodalLDLT::launch_factorize(hipStream_t s) {
+void SupernodalLDLT::solve_fused(double* x, hipStream_t s) {
+  if (sym_.n == 0) return;
+  SYMHIP_CHECK(fused_, "solve_fused needs a factorization given the right-hand side");
+  HIP_OK(hipGraphLaunch(cached_graph(bwd_graphs_, nullptr, x, s, [&] { launch_backward(x, s); }),
+                        s));
+}
+
+template <typename F>
+hipGraphExec_t SupernodalLDLT::cached_graph(std::vector<CachedGraph>& cache, const double* b,
+                                            double* x, hipStream_t s, F&& launches) {
+  for (const auto& g : cache)
+    if (g.b == b && g.x == x) return g.exec;
+  if (cache.size() >= 8) {  // bound the cache; the oldest goes
+    (void)hipGraphExecDestroy(cache.front().exec);
+    cache.erase(cache.begin());
+  }
+  cache.push_back({b, x, capture(s, launches)});
+  return cache.back().exec;
+}
+
+void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
   const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
+  if (b)
+    hipLaunchKernelGGL(permute_kernel, dim3(grid_for(sym_.n, 256)), dim3(256), 0, s, sym_.n,
+                       perm_.get(), b, y_.get(), part_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
     const int nt = lvl_trsm_ptr_[l + 1] - lvl_trsm_ptr_[l];
-    if (nt)
-      hipLaunchKernelGGL(diag_trsm_kernel, dim3(nt), dim3(256), 0, s,
-                         trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(), l11_.get(),
-                         D_.get());
+    if (nt) {
+      if (b)
+        hipLaunchKernelGGL(diag_trsm_kernel<true>, dim3(nt), dim3(kDiagThreads), 0, s,
+                           trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(),
+                           l11_.get(), D_.get(), rows_.get(), y_.get(), yout_.get());
+      else
+        hipLaunchKernelGGL(diag_trsm_kernel<false>, dim3(nt), dim3(kDiagThreads), 0, s,
+                           trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(),
+                           l11_.get(), D_.get(), nullptr, nullptr, nullptr);
+    }
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
     if (nu)
       hipLaunchKernelGGL(update_kernel, dim3(nu), dim3(256), 0, s,
@@ -661,6 +709,13 @@ void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
                        solve_items_.get() + 2 * lvl_solve_ptr_[l], desc, rows_.get(),
                        panel_.get(), l11_.get(), y_.get(), yout_.get());
   }
+  launch_backward(x, s);
+}
+
+// D^-1, the backward levels and the inverse permutation, from the forward result in yout.
+void SupernodalLDLT::launch_backward(double* x, hipStream_t s) {
+  const int n = sym_.n;
+  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
   hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(),
                      yout_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {

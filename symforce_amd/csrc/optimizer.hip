@@ -204,10 +204,11 @@ class CscBackend final : public SystemBackend {
   void solve(int b, double* update, hipStream_t s) override {
     {
       PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
-      chol_.factorize(hval_[b].get(), s);
+      // the forward solve runs inside the factorization
+      chol_.factorize(hval_[b].get(), s, rhs_[b].get());
     }
     PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
-    chol_.solve(rhs_[b].get(), x_.get(), s);
+    chol_.solve_fused(x_.get(), s);
     hipLaunchKernelGGL(negate_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_, x_.get(),
                        update);
   }

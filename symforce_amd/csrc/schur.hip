@@ -1231,11 +1231,12 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
   };
   auto direct = [&]() {
     {
+      // the forward solve runs inside the factorization (v is final by now)
       PhaseScope ps(timer_, SYMHIP_PHASE_FACTOR, s);
-      chol_.factorize_assembled(s);
+      chol_.factorize_assembled(s, v_.get());
     }
     PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
-    chol_.solve(v_.get(), y_.get(), s);
+    chol_.solve_fused(y_.get(), s);
   };
   const bool pcg_now = use_pcg_ && !exact_now_;
   {
