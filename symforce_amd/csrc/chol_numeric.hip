@@ -494,14 +494,17 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ it
       if (lane == 0 && k < n) unsafeAtomicAdd(&part[sd.col_begin + k], v);
     }
   }
+  // Every wave waits for its own atomics into part before the block arrives: a workgroup
+  // barrier does not wait for the memory operations of the other waves, and an agent fence in
+  // the arriving thread would cover only its own wave (and write back and invalidate the L2).
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(&counter[s], 1) == nchunks - 1;
-  }
+  if (threadIdx.x == 0)
+    last = __hip_atomic_fetch_add(&counter[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           nchunks - 1;
   __syncthreads();
   if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
-  __threadfence();
+  // every chunk's sums are in part: read them at agent scope (past the per-XCD L2)
   if (threadIdx.x < 64) {
     const int r = threadIdx.x;
     double yi = 0;
