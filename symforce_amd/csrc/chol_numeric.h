@@ -59,10 +59,10 @@ class SupernodalLDLT {
   DBuf<int32_t> counter_;  // per-supernode arrival counters of the backward pass (self-resetting)
   DBuf<int64_t> asm_map_, scat_col_;
   DBuf<int32_t> rows_, perm_, upd_rel_, scat_rel_;
-  DBuf<uint8_t> desc_;  // per-supernode descriptors (chol_numeric.hip)
+  DBuf<uint8_t> trsm_tasks_, upd_tasks_, solve_tasks_;  // per-level task lists (chol_numeric.hip)
   // work lists (flattened over levels)
   std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_, lvl_solve_ptr_;
-  DBuf<int32_t> lvl_snodes_, trsm_items_, upd_items_, solve_items_;
+  DBuf<int32_t> lvl_snodes_;
 };
 
 }  // namespace symhip

@@ -27,6 +27,14 @@ struct SnDesc {
   int32_t ncols, ld, col_begin, rows_off;
 };
 
+// One task of a level launch: its supernode's descriptor together with the task's own indices,
+// fetched in one scalar-load round trip (an index list followed by a descriptor lookup costs two
+// dependent round trips at the head of every latency-bound launch).
+struct SnTask {
+  SnDesc d;
+  int32_t s, a, b, pad;  // supernode; chunk or tile row; tile column
+};
+
 __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
                                 const double* __restrict__ vals, double* __restrict__ panel) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz;
@@ -131,8 +139,8 @@ __device__ __forceinline__ double lane_value(double v, int k) {
 // y(row) -= L21(row, :) y_own atomically (siblings share ancestors). The same work as a separate
 // forward launch per level, without the launch.
 template <bool kFwd>
-__device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __restrict__ desc,
-                                          double* __restrict__ panel, double* __restrict__ l11,
+__device__ __forceinline__ void diag_body(const SnTask& tk, double* __restrict__ panel,
+                                          double* __restrict__ l11,
                                           double* __restrict__ D, const int32_t* __restrict__ rows,
                                           double* __restrict__ y, double* __restrict__ yout,
                                           double* smem) {
@@ -140,7 +148,8 @@ __device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __rest
   auto cb = reinterpret_cast<double (*)[kCbLd]>(smem);
   auto Wt = reinterpret_cast<double (*)[kDiagRows]>(smem + kStep * kCbLd);
   auto Lc = reinterpret_cast<double (*)[kLcLd]>(smem + kStep * kCbLd + kStep * kDiagRows);
-  const SnDesc sd = desc[s];
+  const SnDesc sd = tk.d;
+  const int chunk = tk.a;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   double* P = panel + sd.panel_off;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -229,15 +238,17 @@ __device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __rest
 #pragma unroll
     for (int k = 0; k < kMaxCols; ++k) acc += Lc[k][kMaxCols + r] * lane_value(yi, k);
     if (chunk * kMaxCols + r < mb) unsafeAtomicAdd(&y[y_row], -acc);
+    return;
   }
-  // write back with coalesced stores: L11 (strict lower, column-major n x n) and D from chunk 0,
-  // the chunk's L21 rows into the panel (the thread index is passed through an empty asm so the
-  // compiler recomputes these addresses here instead of holding the load phase's across the sweep)
-  int tid_w = tid;
+  // Write back with coalesced stores (with the forward solve, wave 0 is busy with it and the
+  // other waves store): L11 (strict lower, column-major n x n) and D from chunk 0, the chunk's
+  // L21 rows into the panel. The thread index passes through an empty asm so the compiler
+  // recomputes these addresses here instead of holding the load phase's across the sweep.
+  constexpr int kStorers = kFwd ? kDiagThreads - 64 : kDiagThreads;
+  int tid_w = kFwd ? tid - 64 : tid;
   asm volatile("" : "+v"(tid_w));
-#pragma unroll
-  for (int q = 0; q < kPer; ++q) {
-    const int idx = tid_w + kDiagThreads * q;
+#pragma unroll 4
+  for (int idx = tid_w; idx < kDiagRows * kMaxCols; idx += kStorers) {
     const int i = idx & (kDiagRows - 1), j = idx / kDiagRows;
     const double v = Lc[j][i];
     if (i < kMaxCols) {
@@ -254,28 +265,26 @@ __device__ __forceinline__ void diag_body(int s, int chunk, const SnDesc* __rest
 
 template <bool kFwd>
 __global__ void __launch_bounds__(kDiagThreads) diag_trsm_kernel(
-    const int32_t* __restrict__ items, const SnDesc* __restrict__ desc, double* __restrict__ panel,
-    double* __restrict__ l11, double* __restrict__ D, const int32_t* __restrict__ rows,
-    double* __restrict__ y, double* __restrict__ yout) {
+    const SnTask* __restrict__ tasks, double* __restrict__ panel, double* __restrict__ l11,
+    double* __restrict__ D, const int32_t* __restrict__ rows, double* __restrict__ y,
+    double* __restrict__ yout) {
   __shared__ double smem[kDiagSmem];
-  diag_body<kFwd>(items[2 * blockIdx.x], items[2 * blockIdx.x + 1], desc, panel, l11, D, rows, y,
-                  yout, smem);
+  diag_body<kFwd>(tasks[blockIdx.x], panel, l11, D, rows, y, yout, smem);
 }
 
 // Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj), scattered as -U
 // into ancestor panels. For below-row j of the supernode, scat_col[j] is the panel offset of its
 // column in the target supernode and scat_rel[j] + i indexes upd_rel for the target row position
 // of below-row i >= j (i < mb).
-__device__ __forceinline__ void update_body(int s, int ti, int tj,
-                                            const SnDesc* __restrict__ desc,
-                                            double* __restrict__ panel,
+__device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict__ panel,
                                             const double* __restrict__ D,
                                             const int64_t* __restrict__ scat_col,
                                             const int32_t* __restrict__ scat_rel,
                                             const int32_t* __restrict__ upd_rel, double* smem) {
   auto As = reinterpret_cast<double (*)[kTile + 1]>(smem);
   auto Bs = reinterpret_cast<double (*)[kTile + 1]>(smem + kMaxCols * (kTile + 1));
-  const SnDesc sd = desc[s];
+  const SnDesc sd = tk.d;
+  const int ti = tk.a, tj = tk.b;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const double* P = panel + sd.panel_off;
   const double* Ds = D + sd.col_begin;
@@ -348,16 +357,14 @@ __device__ __forceinline__ void update_body(int s, int ti, int tj,
   }
 }
 
-__global__ void __launch_bounds__(256) update_kernel(const int32_t* __restrict__ items,
-                                                     const SnDesc* __restrict__ desc,
+__global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ tasks,
                                                      double* __restrict__ panel,
                                                      const double* __restrict__ D,
                                                      const int64_t* __restrict__ scat_col,
                                                      const int32_t* __restrict__ scat_rel,
                                                      const int32_t* __restrict__ upd_rel) {
   __shared__ double smem[kUpdSmem];
-  update_body(items[3 * blockIdx.x], items[3 * blockIdx.x + 1], items[3 * blockIdx.x + 2], desc,
-              panel, D, scat_col, scat_rel, upd_rel, smem);
+  update_body(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
 // y = P b; also clears the backward pass's partial sums.
@@ -400,8 +407,7 @@ constexpr int kSolveRows = 256;  // below-rows per solve block (one per thread i
 // level share ancestors). A thread takes one below-row and all 64 (padded) columns; its 64 panel
 // loads are issued before the triangular solve so they overlap it. Columns past n are clamped to
 // column n - 1 and multiplied by ys = 0.
-__global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ items,
-                                                  const SnDesc* __restrict__ desc,
+__global__ void __launch_bounds__(256) fwd_kernel(const SnTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
                                                   const double* __restrict__ l11,
@@ -409,8 +415,9 @@ __global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ it
                                                   double* __restrict__ yout) {
   __shared__ double Ls[kMaxCols][kMaxCols + 1];
   __shared__ double ys[kMaxCols];
-  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
-  const SnDesc sd = desc[s];
+  const SnTask tk = tasks[blockIdx.x];
+  const int chunk = tk.a;
+  const SnDesc sd = tk.d;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const double* P = panel + sd.panel_off;
   const int32_t* rw = rows + sd.rows_off + n;
@@ -447,8 +454,7 @@ __global__ void __launch_bounds__(256) fwd_kernel(const int32_t* __restrict__ it
 // wave), added atomically into part; the last chunk block of s to finish (a per-supernode arrival
 // counter, reset by that block for the next solve) then forms y_own = L11^-T (y_own - part) with
 // one wavefront. part must be zero on entry (the permutation kernel clears it).
-__global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ items,
-                                                  const SnDesc* __restrict__ desc,
+__global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
                                                   const double* __restrict__ l11,
@@ -457,8 +463,9 @@ __global__ void __launch_bounds__(256) bwd_kernel(const int32_t* __restrict__ it
                                                   int32_t* __restrict__ counter) {
   __shared__ double Ls[kMaxCols][kMaxCols + 1];
   __shared__ int last;
-  const int s = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
-  const SnDesc sd = desc[s];
+  const SnTask tk = tasks[blockIdx.x];
+  const int s = tk.s, chunk = tk.a;
+  const SnDesc sd = tk.d;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const int nchunks = max(1, (mb + kSolveRows - 1) / kSolveRows);
   const double* P = panel + sd.panel_off + n;
@@ -547,9 +554,6 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   }
   l11_.resize(std::max<int64_t>(1, l11_size));
   fill_zero(l11_.get(), l11_.size(), s);
-  desc_.resize(desc.size() * sizeof(SnDesc));
-  HIP_OK(hipMemcpyAsync(desc_.get(), desc.data(), desc.size() * sizeof(SnDesc),
-                        hipMemcpyHostToDevice, s));
   // scatter targets per below-row
   std::vector<int64_t> scol(std::max<size_t>(1, sym.sn_rows.size()), 0);
   std::vector<int32_t> srel(std::max<size_t>(1, sym.sn_rows.size()), 0);
@@ -607,9 +611,18 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
     lvl_upd_ptr_.push_back((int32_t)upd.size() / 3);
   }
   lvl_snodes_.upload(snl, s);
-  solve_items_.upload(sol, s);
-  trsm_items_.upload(dt, s);
-  upd_items_.upload(upd, s);
+  auto upload_tasks = [&](const std::vector<int32_t>& items, int per, DBuf<uint8_t>& out) {
+    std::vector<SnTask> t(std::max<size_t>(1, items.size() / per));
+    for (size_t q = 0; q < items.size() / per; ++q) {
+      const int32_t* it = &items[q * per];
+      t[q] = SnTask{desc[it[0]], it[0], it[1], per > 2 ? it[2] : 0, 0};
+    }
+    out.resize(t.size() * sizeof(SnTask));
+    HIP_OK(hipMemcpyAsync(out.get(), t.data(), t.size() * sizeof(SnTask), hipMemcpyHostToDevice, s));
+  };
+  upload_tasks(sol, 2, solve_tasks_);
+  upload_tasks(dt, 2, trsm_tasks_);
+  upload_tasks(upd, 3, upd_tasks_);
   HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -676,7 +689,8 @@ hipGraphExec_t SupernodalLDLT::cached_graph(std::vector<CachedGraph>& cache, con
 }
 
 void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
-  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
+  const SnTask* trsm = reinterpret_cast<const SnTask*>(trsm_tasks_.get());
+  const SnTask* upd = reinterpret_cast<const SnTask*>(upd_tasks_.get());
   if (b)
     hipLaunchKernelGGL(permute_kernel, dim3(grid_for(sym_.n, 256)), dim3(256), 0, s, sym_.n,
                        perm_.get(), b, y_.get(), part_.get());
@@ -685,17 +699,17 @@ void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
     if (nt) {
       if (b)
         hipLaunchKernelGGL(diag_trsm_kernel<true>, dim3(nt), dim3(kDiagThreads), 0, s,
-                           trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(),
+                           trsm + lvl_trsm_ptr_[l], panel_.get(),
                            l11_.get(), D_.get(), rows_.get(), y_.get(), yout_.get());
       else
         hipLaunchKernelGGL(diag_trsm_kernel<false>, dim3(nt), dim3(kDiagThreads), 0, s,
-                           trsm_items_.get() + 2 * lvl_trsm_ptr_[l], desc, panel_.get(),
+                           trsm + lvl_trsm_ptr_[l], panel_.get(),
                            l11_.get(), D_.get(), nullptr, nullptr, nullptr);
     }
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
     if (nu)
       hipLaunchKernelGGL(update_kernel, dim3(nu), dim3(256), 0, s,
-                         upd_items_.get() + 3 * lvl_upd_ptr_[l], desc, panel_.get(), D_.get(),
+                         upd + lvl_upd_ptr_[l], panel_.get(), D_.get(),
                          scat_col_.get(), scat_rel_.get(), upd_rel_.get());
   }
   HIP_OK(hipGetLastError());
@@ -703,13 +717,13 @@ void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
 
 void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
   const int n = sym_.n;
-  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
+  const SnTask* tasks = reinterpret_cast<const SnTask*>(solve_tasks_.get());
   hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(), b,
                      y_.get(), part_.get());
   for (int l = 0; l < sym_.n_levels; ++l) {
     const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
     hipLaunchKernelGGL(fwd_kernel, dim3(nb), dim3(256), 0, s,
-                       solve_items_.get() + 2 * lvl_solve_ptr_[l], desc, rows_.get(),
+                       tasks + lvl_solve_ptr_[l], rows_.get(),
                        panel_.get(), l11_.get(), y_.get(), yout_.get());
   }
   launch_backward(x, s);
@@ -718,13 +732,13 @@ void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
 // D^-1, the backward levels and the inverse permutation, from the forward result in yout.
 void SupernodalLDLT::launch_backward(double* x, hipStream_t s) {
   const int n = sym_.n;
-  const SnDesc* desc = reinterpret_cast<const SnDesc*>(desc_.get());
+  const SnTask* tasks = reinterpret_cast<const SnTask*>(solve_tasks_.get());
   hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(),
                      yout_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {
     const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
     hipLaunchKernelGGL(bwd_kernel, dim3(nb), dim3(256), 0, s,
-                       solve_items_.get() + 2 * lvl_solve_ptr_[l], desc, rows_.get(),
+                       tasks + lvl_solve_ptr_[l], rows_.get(),
                        panel_.get(), l11_.get(), yout_.get(), part_.get(), counter_.get());
   }
   hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(),
