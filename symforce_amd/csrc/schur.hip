@@ -513,25 +513,62 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
 }
 
 // Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
-// update vector in reference state order.
+// update vector in reference state order. A wavefront takes 64 consecutive points, whose
+// observations are one contiguous range: lanes take consecutive observations (coalesced Z rows
+// instead of one thread walking each point's rows), stage Z_o^T y_node(o) in LDS a chunk at a
+// time, and each lane then sums the observations of its own point.
+constexpr int kBackSubChunk = 128;  // observations staged per wavefront and pass
+
 template <int ND, typename TW>
-__global__ void __launch_bounds__(128) back_substitute_kernel(
+__global__ void __launch_bounds__(256) back_substitute_kernel(
     int32_t n_points, const int32_t* __restrict__ pt_obs_ptr, const int32_t* __restrict__ obs_node,
     const TW* __restrict__ Z, const double* __restrict__ Lp, const double* __restrict__ hp,
     const double* __restrict__ y, const int32_t* __restrict__ pt_tan, double* __restrict__ update) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_points) return;
-  double t0 = hp[p * 3], t1 = hp[p * 3 + 1], t2 = hp[p * 3 + 2];
-  for (int o = pt_obs_ptr[p]; o < pt_obs_ptr[p + 1]; ++o) {
-    const TW* Zo = Z + (int64_t)o * z_stride<ND, TW>();
-    const double* yn = y + obs_node[o] * ND;
+  constexpr int ZS = z_stride<ND, TW>();
+  __shared__ double ts[4][kBackSubChunk][3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p0 = (blockIdx.x * 4 + wave) * 64;
+  if (p0 >= n_points) return;  // wave-uniform; the kernel has no block barrier
+  const int p = p0 + lane, pe = min(p0 + 64, n_points);
+  const int o_begin = pt_obs_ptr[p0], o_end = pt_obs_ptr[pe];
+  const int pc = min(p, n_points - 1);
+  const int my_b = pt_obs_ptr[pc], my_e = p < n_points ? pt_obs_ptr[pc + 1] : my_b;
+  double t0 = 0, t1 = 0, t2 = 0;
+  for (int c0 = o_begin; c0 < o_end; c0 += kBackSubChunk) {
 #pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      t0 -= (double)Zo[i * 3] * yn[i];
-      t1 -= (double)Zo[i * 3 + 1] * yn[i];
-      t2 -= (double)Zo[i * 3 + 2] * yn[i];
+    for (int q = lane; q < kBackSubChunk; q += 64) {
+      const int o = min(c0 + q, o_end - 1);  // clamped; rows past the range are never summed
+      const TW* Zo = Z + (int64_t)o * ZS;
+      const double* yn = y + obs_node[o] * ND;
+      double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const double yi = yn[i];
+        s0 += (double)Zo[i * 3] * yi;
+        s1 += (double)Zo[i * 3 + 1] * yi;
+        s2 += (double)Zo[i * 3 + 2] * yi;
+      }
+      ts[wave][q][0] = s0;
+      ts[wave][q][1] = s1;
+      ts[wave][q][2] = s2;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int lo = max(my_b, c0), hi = min(my_e, c0 + kBackSubChunk);
+    for (int o = lo; o < hi; ++o) {
+      t0 += ts[wave][o - c0][0];
+      t1 += ts[wave][o - c0][1];
+      t2 += ts[wave][o - c0][2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next pass overwrites the staging rows
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  if (p >= n_points) return;
+  t0 = hp[p * 3] - t0;
+  t1 = hp[p * 3 + 1] - t1;
+  t2 = hp[p * 3 + 2] - t2;
   const double* L = Lp + (int64_t)p * 6;
   const double z2 = t2 / L[5];
   const double z1 = (t1 - L[4] * z2) / L[2];
@@ -1301,8 +1338,8 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
                      n_nodes_ * ND, node_tan_.get(), y_.get(), update);
   with_storage(b, [&](auto*, auto* Z) {
     using TW = std::remove_pointer_t<decltype(Z)>;
-    hipLaunchKernelGGL((back_substitute_kernel<ND, TW>), dim3(grid_for(n_points_, 128)),
-                       dim3(128), 0, s, n_points_, pt_obs_ptr_.get(), obs_node_.get(), Z,
+    hipLaunchKernelGGL((back_substitute_kernel<ND, TW>), dim3(grid_for(n_points_, 256)),
+                       dim3(256), 0, s, n_points_, pt_obs_ptr_.get(), obs_node_.get(), Z,
                        Lp_.get(), hp_.get(), y_.get(), pt_tan_.get(), update);
   });
   HIP_OK(hipGetLastError());
