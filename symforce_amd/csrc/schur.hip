@@ -131,7 +131,24 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
   }
 }
 
-// Per camera node (wavefront): recompute its observations' node Jacobians and reduce the node
+// Sums NV per-lane values over the 256 threads of a block (shuffles within each wavefront, then
+// the four wavefront partials through LDS); the totals are valid in threads 0 .. NV - 1 as the
+// return value (zero elsewhere).
+template <int NV>
+__device__ __forceinline__ double block_sum_256(double (&v)[NV], double (*red)[NV]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+  __syncthreads();
+  const int t = threadIdx.x;
+  return t < NV ? red[0][t] + red[1][t] + red[2][t] + red[3][t] : 0.0;
+}
+
+// Per camera node (a block): recompute its observations' node Jacobians and reduce the node
 // diagonal block (full ND x ND) and node rhs. Recomputing is cheaper than storing J.
 template <int FTC, int ND>
 __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
@@ -142,12 +159,15 @@ __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
                                                        double* __restrict__ gB) {
   using F = FT<FTC>;
   constexpr int NH = ND * (ND + 1) / 2;
-  // neighbouring cameras share landmarks: keep consecutive cameras on one XCD
-  const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (wave >= n_nodes) return;
-  double h[NH] = {}, g[ND] = {};
-  for (int q0 = cam_obs_ptr[wave] + lane; q0 < cam_obs_ptr[wave + 1]; q0 += 64) {
+  // a block per camera (its observations spread over four wavefronts); neighbouring cameras
+  // share landmarks: keep consecutive cameras on one XCD
+  __shared__ double red[4][NH + ND];
+  const int cam = xcd_tile(blockIdx.x, n_nodes);
+  if (cam >= n_nodes) return;  // block-uniform
+  double hg[NH + ND] = {};
+  double* h = hg;
+  double* g = hg + NH;
+  for (int q0 = cam_obs_ptr[cam] + threadIdx.x; q0 < cam_obs_ptr[cam + 1]; q0 += 256) {
     const int o = cam_obs[q0];
     const double* a[F::NA];
 #pragma unroll
@@ -168,25 +188,18 @@ __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < NH; ++i)
-    for (int off = 32; off > 0; off >>= 1) h[i] += __shfl_down(h[i], off, 64);
-#pragma unroll
-  for (int i = 0; i < ND; ++i)
-    for (int off = 32; off > 0; off >>= 1) g[i] += __shfl_down(g[i], off, 64);
-  if (lane == 0) {
-    double* Bd = Bdiag + (int64_t)wave * ND * ND;
-    int idx = 0;
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        Bd[i * ND + j] = h[idx];
-        Bd[j * ND + i] = h[idx];
-        ++idx;
-      }
-#pragma unroll
-    for (int i = 0; i < ND; ++i) gB[(int64_t)wave * ND + i] = g[i];
+  const double tot = block_sum_256<NH + ND>(hg, red);
+  // thread t < NH holds packed lower entry t of B, thread NH + i entry i of g
+  const int t = threadIdx.x;
+  if (t < NH) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int j = t - i * (i + 1) / 2;
+    double* Bd = Bdiag + (int64_t)cam * ND * ND;
+    Bd[i * ND + j] = tot;
+    Bd[j * ND + i] = tot;
+  } else if (t < NH + ND) {
+    gB[(int64_t)cam * ND + t - NH] = tot;
   }
 }
 
@@ -1190,7 +1203,7 @@ void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
         });
       }
       PhaseScope ps(timer_, SYMHIP_PHASE_LIN_CAMS, s);
-      hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
+      hipLaunchKernelGGL((lin_cams_kernel<C, ND>), dim3(xcd_grid(n_nodes_)),
                          dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(), ov, values,
                          Bdiag_[b].get(), gB_[b].get());
     }
