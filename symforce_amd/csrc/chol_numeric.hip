@@ -43,28 +43,31 @@ __global__ void assemble_kernel(int64_t nnz, const int64_t* __restrict__ map,
 }
 
 // Diagonal task (s, chunk): one 512-thread block eliminates the 64 x 64 diagonal block of
-// supernode s together with the chunk's 64 below rows in one right-looking sweep of 8-column
+// supernode s together with the chunk's 64 below rows in one right-looking sweep of kStep-column
 // steps (the diagonal block is factored redundantly by every chunk of s; chunk 0 writes L11 to
 // l11 and D), leaving L21 = A21 L11^-T D^-1 for its rows, written back into the panel in place.
 //
 // The 128 x 64 working matrix (rows 0..63 the diagonal block padded with identity, rows 64..127
 // the chunk's below rows) lives in matrix-core accumulators: wave w holds rows 16w .. 16w + 15 as
-// four 16 x 16 tiles. Step k0 = 0, 8, .., 56:
-//  1. the lanes holding columns k0 .. k0 + 7 publish them to cb (LDS);
-//  2. row owners (thread t < 128 owns row t) factor the 8 x 8 pivot block (redundantly, a chain
-//     of 8 reciprocals), form their row's w = c L8^-T (c = the row's published entries) and
-//     L = w D^-1, and write w to Wt and L to Lc (the pivot rows get L8 and D instead);
+// four 16 x 16 tiles. Step k0 = 0, kStep, .., 64 - kStep:
+//  1. the lanes holding columns k0 .. k0 + kStep - 1 publish them to cb (LDS);
+//  2. row owners (thread t < 128 owns row t) factor the kStep x kStep pivot block (redundantly,
+//     a chain of kStep reciprocals), form their row's w = c Lp^-T (c = the row's published
+//     entries, Lp the pivot block's unit lower factor) and L = w D^-1, and write w to Wt and L
+//     to Lc (the pivot rows get Lp and D instead);
 //  3. every wave applies A(i, j) -= sum_u L(i, u) w(j, u) to its live tiles with
 //     v_mfma_f64_16x16x4f64 (operands read straight from Lc and Wt).
-// Each row's elimination is computed once, and an update needs 10 LDS reads per lane per step
-// (a register-patch FMA form read 64, and was bound by them). Tiles that are dead for the rest of
-// the sweep are skipped (the step loop is unrolled; the row test is wave-uniform): columns below
-// k0 + 8 and diagonal-block row tiles above it. Within live tiles nothing is masked: entries
+// Each row's elimination is computed once, and an update needs a few LDS reads per lane per step
+// (a register-patch FMA form read 16 per column, and was bound by them). The owner phase is a
+// dependent FP64 chain that grows with the cube of kStep per step: 4-column steps beat 8-column
+// ones (measured), the extra barriers notwithstanding. Tiles that are dead for the rest of the
+// sweep are skipped (the step loop is unrolled; the row test is wave-uniform): columns below
+// k0 + kStep and diagonal-block row tiles above it. Within live tiles nothing is masked: entries
 // above the diagonal and finished columns pick up products of other such entries but are never
 // read again (the final values are in Lc), and the entries still to be eliminated only ever
 // combine lower-triangle values.
 constexpr int kDiagThreads = 512;
-constexpr int kStep = 8;                   // columns eliminated per sweep step
+constexpr int kStep = 4;                   // columns eliminated per sweep step (4 or 8)
 constexpr int kDiagRows = 2 * kMaxCols;    // diagonal block + one chunk of below rows
 // LDS layouts, padded so the owners' row-wise writes, the publishing lanes' writes and the
 // matrix-core operand reads are free of bank conflicts (row-major arrays with 8-double rows cost
@@ -86,25 +89,25 @@ __device__ __forceinline__ double refined_rcp(double d) {
 // Lc[k0 + u][t] = L(t, k0 + u).
 __device__ __forceinline__ void sweep_owner(int k0, int t, const double (*cb)[kCbLd],
                                             double (*Wt)[kDiagRows], double (*Lc)[kLcLd]) {
-  double d[kStep], dinv[kStep], L8[kStep][kStep];
+  double d[kStep], dinv[kStep], Lp[kStep][kStep];
   // pivot block, column by column: with wp_uv = L_uv d_v (row u's, formed when column u starts),
   // d_u = A_uu - sum_v wp_uv L_uv and L_ru = (A_ru - sum_v L_rv wp_uv) / d_u
 #pragma unroll
   for (int u = 0; u < kStep; ++u) {
     double wp[kStep];
 #pragma unroll
-    for (int v = 0; v < u; ++v) wp[v] = L8[u][v] * d[v];
+    for (int v = 0; v < u; ++v) wp[v] = Lp[u][v] * d[v];
     double dd = cb[u][k0 + u];
 #pragma unroll
-    for (int v = 0; v < u; ++v) dd -= wp[v] * L8[u][v];
+    for (int v = 0; v < u; ++v) dd -= wp[v] * Lp[u][v];
     d[u] = dd;
     dinv[u] = refined_rcp(dd);
 #pragma unroll
     for (int r = u + 1; r < kStep; ++r) {
       double x = cb[u][k0 + r];
 #pragma unroll
-      for (int v = 0; v < u; ++v) x -= L8[r][v] * wp[v];
-      L8[r][u] = x * dinv[u];
+      for (int v = 0; v < u; ++v) x -= Lp[r][v] * wp[v];
+      Lp[r][u] = x * dinv[u];
     }
   }
   double w[kStep], l[kStep];
@@ -112,11 +115,11 @@ __device__ __forceinline__ void sweep_owner(int k0, int t, const double (*cb)[kC
   for (int u = 0; u < kStep; ++u) {
     double c = cb[u][t];
 #pragma unroll
-    for (int v = 0; v < u; ++v) c -= w[v] * L8[u][v];
+    for (int v = 0; v < u; ++v) c -= w[v] * Lp[u][v];
     w[u] = c;
     l[u] = c * dinv[u];
   }
-  // pivot row k0 + r: L8 row left of the diagonal, D on it (w_r = d_r), zero right of it
+  // pivot row k0 + r: Lp row left of the diagonal, D on it (w_r = d_r), zero right of it
   const int r = t - k0;
 #pragma unroll
   for (int u = 0; u < kStep; ++u) {
@@ -213,17 +216,19 @@ __device__ __forceinline__ void diag_body(const SnTask& tk, double* __restrict__
     __syncthreads();
     const int kn = k0 + kStep;  // first column still to be eliminated
     if (kn < kMaxCols && (below_wave || 16 * wave + 15 >= kn)) {
-      // A(i, j) -= sum_u L(i, k0 + u) w(j, u) on the matrix cores, two K = 4 halves per tile:
+      // A(i, j) -= sum_u L(i, k0 + u) w(j, u) on the matrix cores, kStep / 4 K = 4 pieces per tile:
       // lane l feeds A(l % 16, l / 16) = -L(16w + l % 16, k0 + 4h + l / 16) and
       // B(l / 16, l % 16) = w(16c + l % 16, 4h + l / 16)
-      const double a0 = -Lc[k0 + lk][16 * wave + lr];
-      const double a1 = -Lc[k0 + 4 + lk][16 * wave + lr];
+      double av[kStep / 4];
+#pragma unroll
+      for (int h = 0; h < kStep / 4; ++h) av[h] = -Lc[k0 + 4 * h + lk][16 * wave + lr];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (16 * c + 15 < kn) continue;  // tile columns already eliminated
-        const double b0 = Wt[lk][16 * c + lr], b1 = Wt[4 + lk][16 * c + lr];
-        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[c], 0, 0, 0);
-        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[c], 0, 0, 0);
+#pragma unroll
+        for (int h = 0; h < kStep / 4; ++h)
+          acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], Wt[4 * h + lk][16 * c + lr], acc[c],
+                                                         0, 0, 0);
       }
     }
   }
