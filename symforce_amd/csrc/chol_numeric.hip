@@ -18,6 +18,7 @@ namespace {
 
 constexpr int kMaxCols = 64;  // supernode width cap (panels padded to this width)
 constexpr int kTile = 64;     // update tile (rows)
+constexpr int kSmallLevelTiles = 256;  // below this many 64-row tiles a level uses 32-row tiles
 
 // l11_off: the factored diagonal block L11 (n x n, column-major, strictly lower part used) is
 // kept apart from the panel, so the row-chunk blocks of one supernode, which all read the
@@ -277,17 +278,22 @@ __global__ void __launch_bounds__(kDiagThreads) diag_trsm_kernel(
   diag_body<kFwd>(tasks[blockIdx.x], panel, l11, D, rows, y, yout, smem);
 }
 
-// Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj), scattered as -U
-// into ancestor panels. For below-row j of the supernode, scat_col[j] is the panel offset of its
-// column in the target supernode and scat_rel[j] + i indexes upd_rel for the target row position
-// of below-row i >= j (i < mb).
+// Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj) of KT x KT,
+// scattered as -U into ancestor panels. For below-row j of the supernode, scat_col[j] is the
+// panel offset of its column in the target supernode and scat_rel[j] + i indexes upd_rel for the
+// target row position of below-row i >= j (i < mb). Levels with fewer 64-row tiles than CUs use
+// 32-row tiles: a block's product (K = 64 on the FP64 matrix cores, 4096 cycles for a 64-row
+// tile on one CU) is the latency of such a level, and a quarter of it spreads over four times
+// the blocks.
+template <int KT>
 __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict__ panel,
                                             const double* __restrict__ D,
                                             const int64_t* __restrict__ scat_col,
                                             const int32_t* __restrict__ scat_rel,
                                             const int32_t* __restrict__ upd_rel, double* smem) {
-  auto As = reinterpret_cast<double (*)[kTile + 1]>(smem);
-  auto Bs = reinterpret_cast<double (*)[kTile + 1]>(smem + kMaxCols * (kTile + 1));
+  static_assert(KT == 64 || KT == 32, "update tiles are 64 or 32 rows");
+  auto As = reinterpret_cast<double (*)[KT + 1]>(smem);
+  auto Bs = reinterpret_cast<double (*)[KT + 1]>(smem + kMaxCols * (KT + 1));
   const SnDesc sd = tk.d;
   const int ti = tk.a, tj = tk.b;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
@@ -296,72 +302,78 @@ __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict
   // stage both row tiles (all k at once: the loads are independent)
   // (unconditional loads from clamped addresses; an update item exists only when mb > 0)
 #pragma unroll
-  for (int t = 0; t < kMaxCols * kTile / 256; ++t) {  // blockDim.x == 256
+  for (int t = 0; t < kMaxCols * KT / 256; ++t) {  // blockDim.x == 256
     const int idx = threadIdx.x + 256 * t;
-    const int k = idx / kTile, r = idx % kTile;
-    const int ri = ti * kTile + r, rj = tj * kTile + r;
+    const int k = idx / KT, r = idx % KT;
+    const int ri = ti * KT + r, rj = tj * KT + r;
     const int kc = min(k, n - 1);
     const double va = P[(int64_t)kc * ld + n + min(ri, mb - 1)];
     const double vb = P[(int64_t)kc * ld + n + min(rj, mb - 1)] * Ds[kc];
     As[k][r] = (k < n && ri < mb) ? va : 0.0;
     Bs[k][r] = (k < n && rj < mb) ? vb : 0.0;
   }
-  // The product on the FP64 matrix cores: wave w computes rows 16w .. 16w + 15 of the tile as
-  // four 16 x 16 output blocks c (columns 16c ..), K = 64 in 16 steps of v_mfma_f64_16x16x4f64.
-  // Lane l feeds A(i = l % 16, k = l / 16) = As[k][16w + i] and B(k, j = l % 16) = Bs[k][16c + j]
-  // and holds D(l / 16 + 4 r, l % 16) of block c in accumulator register r: 5 LDS reads per 4
-  // MFMAs (the FMA form needed 8 per 16 FMAs and was bound by them).
+  // The product on the FP64 matrix cores, in 16 x 16 output blocks, K = 64 in 16 steps of
+  // v_mfma_f64_16x16x4f64: a 64-row tile gives wave w row block w and all four column blocks, a
+  // 32-row tile gives it row block w / 2 and column block w % 2. Lane l feeds
+  // A(i = l % 16, k = l / 16) = As[k][16 rb + i] and B(k, j = l % 16) = Bs[k][16 c + j] and holds
+  // D(l / 16 + 4 r, l % 16) of block c in accumulator register r.
   typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int NBW = KT == 64 ? 4 : 1;  // output blocks per wave
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar metadata
   const int lr = lane & 15, lk = lane >> 4;
+  const int rb = KT == 64 ? wave : wave >> 1, c0 = KT == 64 ? 0 : wave & 1;
   // Scatter targets, fetched while the tiles load and before the product loop (the row
   // positions depend on the metadata: two dependent loads, off the critical path here). The
-  // scatter runs column by column: wave w takes tile columns w, w + 4, ..., w + 60 and lane l row
-  // l, so one atomic instruction covers 64 consecutive rows of one target column (a few cache
-  // lines) instead of 16 columns x 4 rows. Row indices are clamped into the column's update
-  // segment (rows j .. mb - 1); clamped entries are masked at the scatter.
-  constexpr int kCols = kTile / 4;
-  int64_t cbase[kCols];
-  int32_t rel[kCols];
+  // scatter runs column by column: wave w takes tile columns w, w + 4, .. and each atomic
+  // instruction covers KT consecutive rows of 64 / KT columns (a few cache lines) instead of 16
+  // columns x 4 rows. Row indices are clamped into the column's update segment (rows j ..
+  // mb - 1); clamped entries are masked at the scatter.
+  constexpr int kCpi = 64 / KT;             // columns per scatter instruction
+  constexpr int kNq = KT / 4 / kCpi;        // scatter instructions per lane
+  const int srow = lane % KT, ssub = lane / KT;
+  int64_t cbase[kNq];
+  int32_t rel[kNq];
   {
-    const int i = ti * kTile + lane;
+    const int i = ti * KT + srow;
 #pragma unroll
-    for (int q = 0; q < kCols; ++q) {
-      const int j = min(tj * kTile + wave + 4 * q, mb - 1);
+    for (int q = 0; q < kNq; ++q) {
+      const int j = min(tj * KT + wave + 4 * (kCpi * q + ssub), mb - 1);
       cbase[q] = scat_col[sd.rows_off + n + j];
       const int32_t rbase = scat_rel[sd.rows_off + n + j];
       rel[q] = upd_rel[rbase + min(max(i, j), mb - 1)];
     }
   }
   __syncthreads();
-  d4 acc[4] = {};
+  d4 acc[NBW] = {};
 #pragma unroll 4
   for (int k0 = 0; k0 < kMaxCols; k0 += 4) {
-    const double av = As[k0 + lk][16 * wave + lr];
-    double bv[4];
+    const double av = As[k0 + lk][16 * rb + lr];
+    double bv[NBW];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) bv[c] = Bs[k0 + lk][16 * c + lr];
+    for (int c = 0; c < NBW; ++c) bv[c] = Bs[k0 + lk][16 * (c0 + c) + lr];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
+    for (int c = 0; c < NBW; ++c)
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
   }
   // transpose through LDS (the A tile's space): T[column][row]
   __syncthreads();
   auto T = As;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < NBW; ++c)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) T[16 * c + lr][16 * wave + lk + 4 * r] = acc[c][r];
+    for (int r = 0; r < 4; ++r) T[16 * (c0 + c) + lr][16 * rb + lk + 4 * r] = acc[c][r];
   __syncthreads();
-  const int i = ti * kTile + lane;
+  const int i = ti * KT + srow;
 #pragma unroll
-  for (int q = 0; q < kCols; ++q) {
-    const int jl = wave + 4 * q, j = tj * kTile + jl;
-    const double v = T[jl][lane];
+  for (int q = 0; q < kNq; ++q) {
+    const int jl = wave + 4 * (kCpi * q + ssub), j = tj * KT + jl;
+    const double v = T[jl][srow];
     if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[q] + rel[q], -v);
   }
 }
 
+template <int KT>
 __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ tasks,
                                                      double* __restrict__ panel,
                                                      const double* __restrict__ D,
@@ -369,7 +381,7 @@ __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ 
                                                      const int32_t* __restrict__ scat_rel,
                                                      const int32_t* __restrict__ upd_rel) {
   __shared__ double smem[kUpdSmem];
-  update_body(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
+  update_body<KT>(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
 // y = P b; also clears the backward pass's partial sums.
@@ -586,7 +598,9 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   lvl_solve_ptr_.assign(1, 0);
   lvl_trsm_ptr_.assign(1, 0);
   lvl_upd_ptr_.assign(1, 0);
+  lvl_upd32_.assign(sym.n_levels, 0);
   for (int l = 0; l < sym.n_levels; ++l) {
+    int64_t tiles64 = 0;
     for (int q = sym.level_ptr[l]; q < sym.level_ptr[l + 1]; ++q) {
       const int sn = sym.level_snodes[q];
       snl.push_back(sn);
@@ -603,6 +617,16 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
         sol.push_back(c);
       }
       const int T = (mb + kTile - 1) / kTile;
+      tiles64 += T * (T + 1) / 2;
+    }
+    // update tiles: 32 rows where the 64-row tiles would not fill the CUs
+    const int kt = tiles64 < kSmallLevelTiles ? kTile / 2 : kTile;
+    lvl_upd32_[l] = kt != kTile;
+    for (int q = sym.level_ptr[l]; q < sym.level_ptr[l + 1]; ++q) {
+      const int sn = sym.level_snodes[q];
+      const int n = sym.sn_col_begin[sn + 1] - sym.sn_col_begin[sn];
+      const int mb = sym.sn_rows_ptr[sn + 1] - sym.sn_rows_ptr[sn] - n;
+      const int T = (mb + kt - 1) / kt;
       for (int ti = 0; ti < T; ++ti)
         for (int tj = 0; tj <= ti; ++tj) {
           upd.push_back(sn);
@@ -712,10 +736,16 @@ void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
                            l11_.get(), D_.get(), nullptr, nullptr, nullptr);
     }
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
-    if (nu)
-      hipLaunchKernelGGL(update_kernel, dim3(nu), dim3(256), 0, s,
-                         upd + lvl_upd_ptr_[l], panel_.get(), D_.get(),
-                         scat_col_.get(), scat_rel_.get(), upd_rel_.get());
+    if (nu) {
+      if (lvl_upd32_[l])
+        hipLaunchKernelGGL(update_kernel<kTile / 2>, dim3(nu), dim3(256), 0, s,
+                           upd + lvl_upd_ptr_[l], panel_.get(), D_.get(), scat_col_.get(),
+                           scat_rel_.get(), upd_rel_.get());
+      else
+        hipLaunchKernelGGL(update_kernel<kTile>, dim3(nu), dim3(256), 0, s,
+                           upd + lvl_upd_ptr_[l], panel_.get(), D_.get(), scat_col_.get(),
+                           scat_rel_.get(), upd_rel_.get());
+    }
   }
   HIP_OK(hipGetLastError());
 }
