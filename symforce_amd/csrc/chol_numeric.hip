@@ -416,7 +416,10 @@ __device__ __forceinline__ void stage_l11(const double* Lc, int n, double (*Ls)[
   }
 }
 
-constexpr int kSolveRows = 256;  // below-rows per solve block (one per thread in the forward pass)
+// below-rows per solve block: a backward block sums them 64 at a time (chunks of 128 beat 256 and
+// 512: more, shorter blocks per level); a forward block takes one per thread
+constexpr int kSolveRows = 128;
+static_assert(kSolveRows % 64 == 0 && kSolveRows <= 256, "solve chunk: whole waves, one block");
 
 // Forward, block (s, chunk): y_own = L11^-1 y_own (one wavefront, lane r holds y_r, L11 from LDS;
 // every chunk block of s solves redundantly, chunk 0 writes the result to yout) and, for the
@@ -458,7 +461,7 @@ __global__ void __launch_bounds__(256) fwd_kernel(const SnTask* __restrict__ tas
     ys[r] = r < n ? yi : 0.0;
   }
   __syncthreads();
-  if (i < mb) {
+  if (threadIdx.x < kSolveRows && i < mb) {  // a chunk may be narrower than the block
     double acc = 0;
 #pragma unroll
     for (int k = 0; k < kMaxCols; ++k) acc += lrow[k] * ys[k];
