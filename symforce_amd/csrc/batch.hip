@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "./factor_traits.h"
 #include "./geo.h"
@@ -54,7 +55,13 @@ __device__ __forceinline__ double wave_total(double v) {
 __device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 
 // Relinearize state block b: lane i < n writes pose i's D, H(i + 1, i), rhs. Returns the error
-// 0.5 |r|^2 of the problem (the same value in every lane).
+// 0.5 |r|^2 of the problem (the same value in every lane). With S = float (the precision form)
+// the Jacobians come from the FP32 factor bodies while the residual, and with it the cost and
+// the gradient J^T r, comes from the FP64 bodies: the cost stays FP64-accurate, so LM's gain
+// ratio and relative-reduction exit behave as in FP64 (FP32 residuals put a ~1e-7 floor under
+// the cost, which kept LM from its early exit on some problems). The products J^T J are formed
+// in FP64 from the FP32 Jacobians, so H stays positive semi-definite (FP32 sums of products
+// made a near-singular H slightly indefinite, and the block Cholesky then failed).
 template <typename S>
 __device__ double linearize(const BatchView& v, int b, int64_t g, int lane, bool live) {
   double D[21] = {}, gv[6] = {}, Bv[36] = {}, Dn[21] = {}, gn[6] = {};
@@ -65,60 +72,74 @@ __device__ double linearize(const BatchView& v, int b, int64_t g, int lane, bool
     for (int m = v.match_ptr[g]; m < v.match_ptr[g + 1]; ++m) {
       const double* a[4] = {pose, v.cvals + v.match_args[3 * m], v.cvals + v.match_args[3 * m + 1],
                             v.cvals + v.match_args[3 * m + 2]};
-      S r[3], J[3][6];
-      factor_matching<S>(a, r, J);
+      S rs[3], J[3][6];
+      factor_matching<S>(a, rs, J);
+      double r[3], Jd[3][6];
+      if constexpr (std::is_same_v<S, double>) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) r[q] = rs[q];
+      } else {
+        factor_matching<double>(a, r, Jd);
+      }
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) gv[i] += (double)(J[q][i] * r[q]);
+        for (int q = 0; q < 3; ++q) gv[i] += (double)J[q][i] * r[q];
 #pragma unroll
         for (int j = 0; j <= i; ++j) {
-          S h = 0;
+          double h = 0;
 #pragma unroll
-          for (int q = 0; q < 3; ++q) h += J[q][i] * J[q][j];
-          D[tri(i, j)] += (double)h;
+          for (int q = 0; q < 3; ++q) h += (double)J[q][i] * (double)J[q][j];
+          D[tri(i, j)] += h;
         }
       }
 #pragma unroll
-      for (int q = 0; q < 3; ++q) e += (double)r[q] * (double)r[q];
+      for (int q = 0; q < 3; ++q) e += r[q] * r[q];
     }
     const int32_t* oa = v.odo_args + 3 * g;
     if (oa[0] >= 0) {
       has_next = 1;
       const double* a[5] = {pose, pose + 7, v.cvals + oa[0], v.cvals + oa[1], v.cvals + oa[2]};
-      S r[6], J[6][12];
-      factor_odometry<S>(a, r, J);
+      S rs[6], J[6][12];
+      factor_odometry<S>(a, rs, J);
+      double r[6], Jd[6][12];
+      if constexpr (std::is_same_v<S, double>) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r[q] = rs[q];
+      } else {
+        factor_odometry<double>(a, r, Jd);
+      }
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
-        S ga = 0, gb = 0;
+        double ga = 0, gb = 0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-          ga += J[q][i] * r[q];
-          gb += J[q][6 + i] * r[q];
+          ga += (double)J[q][i] * r[q];
+          gb += (double)J[q][6 + i] * r[q];
         }
-        gv[i] += (double)ga;
-        gn[i] = (double)gb;
+        gv[i] += ga;
+        gn[i] = gb;
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
-          S hb = 0;
+          double hb = 0;
 #pragma unroll
-          for (int q = 0; q < 6; ++q) hb += J[q][6 + i] * J[q][j];
-          Bv[i * 6 + j] = (double)hb;  // H(i + 1, i): rows of pose i + 1, cols of pose i
+          for (int q = 0; q < 6; ++q) hb += (double)J[q][6 + i] * (double)J[q][j];
+          Bv[i * 6 + j] = hb;  // H(i + 1, i): rows of pose i + 1, cols of pose i
         }
 #pragma unroll
         for (int j = 0; j <= i; ++j) {
-          S ha = 0, hn = 0;
+          double ha = 0, hn = 0;
 #pragma unroll
           for (int q = 0; q < 6; ++q) {
-            ha += J[q][i] * J[q][j];
-            hn += J[q][6 + i] * J[q][6 + j];
+            ha += (double)J[q][i] * (double)J[q][j];
+            hn += (double)J[q][6 + i] * (double)J[q][6 + j];
           }
-          D[tri(i, j)] += (double)ha;
-          Dn[tri(i, j)] = (double)hn;
+          D[tri(i, j)] += ha;
+          Dn[tri(i, j)] = hn;
         }
       }
 #pragma unroll
-      for (int q = 0; q < 6; ++q) e += (double)r[q] * (double)r[q];
+      for (int q = 0; q < 6; ++q) e += r[q] * r[q];
     }
   }
   // the odometry factor's pose-(i + 1) terms belong to the next lane

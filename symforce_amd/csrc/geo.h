@@ -117,6 +117,9 @@ SYMHIP_HD M3<S> skew(V3<S> v) {
 template <typename S>
 SYMHIP_HD void rot_log(const Quat<S>& q, S eps, V3<S>& out, M3<S>& D) {
   const S aw = q.w < 0 ? -q.w : q.w;
+  // The clamp must be representable in S: an FP64-sized epsilon rounds 1 - eps to 1 in FP32,
+  // and an exactly-identity relative rotation then gives acos(1) / sqrt(0) = 0 / 0.
+  if constexpr (sizeof(S) == 4) eps = eps > S(8e-7) ? eps : S(8e-7);  // ~7 FP32 epsilons
   const S one_m_eps = S(1) - eps;
   const bool unclamped = aw < one_m_eps;
   const S m = unclamped ? aw : one_m_eps;
