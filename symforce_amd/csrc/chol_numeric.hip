@@ -399,10 +399,6 @@ __global__ void unpermute_kernel(int32_t n, const int32_t* __restrict__ perm,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = y[perm[i]];
 }
-__global__ void dscale_kernel(int32_t n, const double* __restrict__ D, double* __restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] /= D[i];
-}
 
 // Stages the strictly lower part of L11 (n x n column-major at Lc, padded with zeros; unit
 // diagonal implied) into LDS: Ls[i][k].
@@ -472,12 +468,14 @@ __global__ void __launch_bounds__(256) fwd_kernel(const SnTask* __restrict__ tas
 // Backward, block (s, chunk): part(own) += L21(chunk rows, :)^T y(chunk rows) (wave w takes
 // columns w, w + 4, ..., w + 60: 16 accumulators per lane, lanes over the rows, reduced across the
 // wave), added atomically into part; the last chunk block of s to finish (a per-supernode arrival
-// counter, reset by that block for the next solve) then forms y_own = L11^-T (y_own - part) with
-// one wavefront. part must be zero on entry (the permutation kernel clears it).
+// counter, reset by that block for the next solve) then forms y_own = L11^-T (D^-1 y_own - part)
+// with one wavefront (the D^-1 of the solve folded in: own entries are read by their supernode's
+// blocks only). part must be zero on entry (the permutation kernel clears it).
 __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
                                                   const double* __restrict__ l11,
+                                                  const double* __restrict__ D,
                                                   double* __restrict__ yout,
                                                   double* __restrict__ part,
                                                   int32_t* __restrict__ counter) {
@@ -496,7 +494,7 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
   // written by no other block of this launch).
   stage_l11(l11 + sd.l11_off, n, Ls);
   double yown = 0;
-  if (threadIdx.x < n) yown = yout[sd.col_begin + threadIdx.x];
+  if (threadIdx.x < n) yown = yout[sd.col_begin + threadIdx.x] / D[sd.col_begin + threadIdx.x];
   if (mb > 0) {
     constexpr int kPer = kMaxCols / 4;
     double acc[kPer] = {};
@@ -767,17 +765,17 @@ void SupernodalLDLT::launch_solve(const double* b, double* x, hipStream_t s) {
   launch_backward(x, s);
 }
 
-// D^-1, the backward levels and the inverse permutation, from the forward result in yout.
+// The backward levels (D^-1 folded in) and the inverse permutation, from the forward result in
+// yout.
 void SupernodalLDLT::launch_backward(double* x, hipStream_t s) {
   const int n = sym_.n;
   const SnTask* tasks = reinterpret_cast<const SnTask*>(solve_tasks_.get());
-  hipLaunchKernelGGL(dscale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, D_.get(),
-                     yout_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {
     const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
     hipLaunchKernelGGL(bwd_kernel, dim3(nb), dim3(256), 0, s,
                        tasks + lvl_solve_ptr_[l], rows_.get(),
-                       panel_.get(), l11_.get(), yout_.get(), part_.get(), counter_.get());
+                       panel_.get(), l11_.get(), D_.get(), yout_.get(), part_.get(),
+                       counter_.get());
   }
   hipLaunchKernelGGL(unpermute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, perm_.get(),
                      yout_.get(), x);
