@@ -480,6 +480,7 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
                                                   double* __restrict__ part,
                                                   int32_t* __restrict__ counter) {
   __shared__ double Ls[kMaxCols][kMaxCols + 1];
+  __shared__ double red_all[4][(kMaxCols / 4) * (64 + 1)];  // per-wave column partials
   __shared__ int last;
   const SnTask tk = tasks[blockIdx.x];
   const int s = tk.s, chunk = tk.a;
@@ -510,14 +511,23 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
 #pragma unroll
       for (int c = 0; c < kPer; ++c) acc[c] += P[coff[c] + ic] * yv;
     }
+    // Each wave's 16 column sums across its 64 lanes, through LDS: the partials go in as
+    // red[c][lane], lane l then adds 16 of column l / 4's (quarter l % 4) and two quad shuffles
+    // finish. (Six rounds of 64-lane shuffles for 16 values took longer than the loads.)
+    auto red = reinterpret_cast<double (*)[64 + 1]>(&red_all[wave][0]);
 #pragma unroll
-    for (int c = 0; c < kPer; ++c) {
-      double v = acc[c];
+    for (int c = 0; c < kPer; ++c) red[c][lane] = acc[c];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int c = lane >> 2, qtr = lane & 3;
+    double v = 0;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      const int k = wave + 4 * c;
-      if (lane == 0 && k < n) unsafeAtomicAdd(&part[sd.col_begin + k], v);
-    }
+    for (int t = 0; t < 16; ++t) v += red[c][16 * qtr + t];
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    const int k = wave + 4 * c;
+    if (qtr == 0 && k < n) unsafeAtomicAdd(&part[sd.col_begin + k], v);
   }
   // Every wave waits for its own atomics into part before the block arrives: a workgroup
   // barrier does not wait for the memory operations of the other waves, and an agent fence in
