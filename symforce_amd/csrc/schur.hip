@@ -131,18 +131,41 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
   }
 }
 
-// Sums NV per-lane values over the 256 threads of a block (shuffles within each wavefront, then
-// the four wavefront partials through LDS); the totals are valid in threads 0 .. NV - 1 as the
-// return value (zero elsewhere).
+// Sums NV per-lane values over the 64 lanes of a wavefront through LDS, 16 values a round (scr:
+// this wavefront's own 16 x 65 doubles): the partials go in as scr[c][lane], lane l adds 16 of
+// value l / 4's (quarter l % 4) and two quad shuffles finish; out(i, total) runs in the lane
+// with l % 4 == 0 of value i. Six rounds of 64-lane shuffles per value cost several times more.
+template <int NV, typename F>
+__device__ __forceinline__ void wave_sum_lds(const double (&v)[NV], double (*scr)[65], F&& out) {
+  const int lane = threadIdx.x & 63, c = lane >> 2, qtr = lane & 3;
+#pragma unroll
+  for (int g = 0; g < NV; g += 16) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (g + k < NV) scr[k][lane] = v[g + k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t += scr[c][16 * qtr + j];
+    t += __shfl_xor(t, 1, 64);
+    t += __shfl_xor(t, 2, 64);
+    if (qtr == 0 && g + c < NV) out(g + c, t);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next round overwrites scr
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Sums NV per-lane values over the 256 threads of a block (within each wavefront through LDS,
+// then the four wavefront partials); the totals are valid in threads 0 .. NV - 1 as the return
+// value (zero elsewhere).
 template <int NV>
-__device__ __forceinline__ double block_sum_256(double (&v)[NV], double (*red)[NV]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; ++i)
-    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
+__device__ __forceinline__ double block_sum_256(double (&v)[NV], double (*red)[NV],
+                                                double (*scr)[16][65]) {
+  const int wave = threadIdx.x >> 6;
+  wave_sum_lds<NV>(v, scr[wave], [&](int i, double t) { red[wave][i] = t; });
   __syncthreads();
   const int t = threadIdx.x;
   return t < NV ? red[0][t] + red[1][t] + red[2][t] + red[3][t] : 0.0;
@@ -162,6 +185,7 @@ __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
   // a block per camera (its observations spread over four wavefronts); neighbouring cameras
   // share landmarks: keep consecutive cameras on one XCD
   __shared__ double red[4][NH + ND];
+  __shared__ double scr[4][16][65];
   const int cam = xcd_tile(blockIdx.x, n_nodes);
   if (cam >= n_nodes) return;  // block-uniform
   double hg[NH + ND] = {};
@@ -188,7 +212,7 @@ __global__ void __launch_bounds__(256) lin_cams_kernel(int32_t n_nodes,
       }
     }
   }
-  const double tot = block_sum_256<NH + ND>(hg, red);
+  const double tot = block_sum_256<NH + ND>(hg, red, scr);
   // thread t < NH holds packed lower entry t of B, thread NH + i entry i of g
   const int t = threadIdx.x;
   if (t < NH) {
@@ -482,6 +506,7 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           double* __restrict__ v,
                                                           double* __restrict__ ZZ) {
   constexpr int NH = ND * (ND + 1) / 2;
+  __shared__ double scr[4][16][65];  // per-wavefront reduction staging (no block barrier)
   const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wave >= n_nodes) return;
@@ -503,26 +528,24 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
       for (int k = 0; k <= i; ++k)
         zz[idx++] += z[i * 3] * z[k * 3] + z[i * 3 + 1] * z[k * 3 + 1] + z[i * 3 + 2] * z[k * 3 + 2];
   }
+  double az[ND + NH];
 #pragma unroll
-  for (int i = 0; i < ND; ++i)
-    for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_down(acc[i], off, 64);
+  for (int i = 0; i < ND; ++i) az[i] = acc[i];
 #pragma unroll
-  for (int i = 0; i < NH; ++i)
-    for (int off = 32; off > 0; off >>= 1) zz[i] += __shfl_down(zz[i], off, 64);
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) v[wave * ND + i] = gB[wave * ND + i] - acc[i];
-    double* zo = ZZ + (int64_t)wave * ND * ND;
-    int idx = 0;
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-      for (int k = 0; k <= i; ++k) {
-        zo[i * ND + k] = zz[idx];
-        zo[k * ND + i] = zz[idx];
-        ++idx;
-      }
-  }
+  for (int i = 0; i < NH; ++i) az[ND + i] = zz[i];
+  double* zo = ZZ + (int64_t)wave * ND * ND;
+  wave_sum_lds<ND + NH>(az, scr[threadIdx.x >> 6], [&](int e, double t) {
+    if (e < ND) {
+      v[wave * ND + e] = gB[wave * ND + e] - t;
+    } else {
+      const int p = e - ND;  // packed lower entry (i, k), k <= i
+      int i = 0;
+      while ((i + 1) * (i + 2) / 2 <= p) ++i;
+      const int k = p - i * (i + 1) / 2;
+      zo[i * ND + k] = t;
+      zo[k * ND + i] = t;
+    }
+  });
 }
 
 // Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
