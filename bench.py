@@ -157,7 +157,7 @@ def roofline(r):
     """The largest single kernel launch of an LM iteration is the reduced-camera-system assembly
     (one launch; its phase timer brackets exactly that kernel with HIP events on the optimizer's
     stream). It reduces every off-diagonal S block over its observation pairs on the FP64
-    matrix cores (K = 3 per pair, packed densely: 8 pairs per 6 MFMAs): the useful work is
+    matrix cores (K = 3 per pair, packed densely: 4 pairs per 3 MFMAs): the useful work is
     81 entries x 3 FMAs x 2 = 486 flops per pair, and the unique HBM bytes are the pair list, Z
     once and the S blocks out (the diagonal blocks' own-observation sums are reduced per camera
     in the reduced-rhs kernel). The supernodal LDL^T phase (one diagonal+panel launch and one

@@ -372,12 +372,14 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // One wavefront per S block (a, b): S = [Bdiag + damping - ZZ_a if a == b] + Bfac
 // - sum_pairs Z_x Z_y^T. The pair sum is the product X Y^T with X = [Z_x1 Z_x2 ...] (ND x 3P)
 // and Y = [Z_y1 Z_y2 ...], so K = 3P is packed densely into FP64 matrix-core steps
-// (v_mfma_f64_16x16x4f64; eight pairs = 24 K entries = six MFMAs): at step h lane l feeds
-// A(i = l % 16, k = l / 16) = X(i, 4h + k) and B(k, j = l % 16) = Y(j, 4h + k) (rows i >= ND and
-// K entries past the block's pairs are zero) and holds D(l / 16 + 4 r, l % 16) in accumulator
-// register r (the FP64 16x16x4 output interleaves rows across the four lane groups). The eight
-// pairs' observation indices are wave-uniform scalar loads (the pair list is padded so a batch
-// reaching past the last block stays inside it). Diagonal blocks carry no pairs: their
+// (v_mfma_f64_16x16x4f64; a batch of four pairs = 12 K entries = three MFMAs): at step h lane l
+// feeds A(i = l % 16, k = l / 16) = X(i, 4h + k) and B(k, j = l % 16) = Y(j, 4h + k) (rows
+// i >= ND and K entries past the block's pairs are zero) and holds D(l / 16 + 4 r, l % 16) in
+// accumulator register r (the FP64 16x16x4 output interleaves rows across the four lane groups).
+// A batch's observation indices are wave-uniform scalar loads (the pair list is padded so a
+// batch reaching past the last block stays inside it). Four-pair batches beat eight-pair ones
+// (313 vs 326 us at 1000 cameras): 78 instead of 100 registers, six wavefronts per SIMD instead
+// of five, and the kernel is latency-bound. Diagonal blocks carry no pairs: their
 // sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take work items from a
 // host-built list: (block, pair range, flags), blocks with many pairs split into several items
 // whose partial sums are added atomically (kRcsAtomic; the output is zero-filled first) and only
@@ -387,7 +389,7 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // (transposed). Z is read as TW (FP32 in the FP32 form) and widened: the sums are FP64 on the
 // matrix cores either way; the full storage is written as TS.
 constexpr int kRcsChunk = 512;  // pairs per work item at most
-constexpr int kRcsBatch = 8;    // pairs per batch (a multiple of 4: 3 K entries per pair, 4 per MFMA)
+constexpr int kRcsBatch = 4;    // pairs per batch (a multiple of 4: 3 K entries per pair, 4 per MFMA)
 constexpr bool kRcsHeaviestFirst = true;  // order each XCD's items by size (else by camera row)
 constexpr int kRcsFirst = 1, kRcsAtomic = 2;
 template <int ND, typename TW, typename TS>
