@@ -377,9 +377,10 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // i >= ND and K entries past the block's pairs are zero) and holds D(l / 16 + 4 r, l % 16) in
 // accumulator register r (the FP64 16x16x4 output interleaves rows across the four lane groups).
 // A batch's observation indices are wave-uniform scalar loads (the pair list is padded so a
-// batch reaching past the last block stays inside it). Four-pair batches beat eight-pair ones
-// (313 vs 326 us at 1000 cameras): 78 instead of 100 registers, six wavefronts per SIMD instead
-// of five, and the kernel is latency-bound. Diagonal blocks carry no pairs: their
+// batch reaching past the last block stays inside it). The kernel is latency-bound, so registers
+// are what count: four-pair batches into one accumulator (44 VGPRs, eight wavefronts per SIMD)
+// take 296 us at 1000 cameras, against 326 us for eight-pair batches into three accumulators
+// (100 VGPRs, five wavefronts) and 320 us for eight pairs into one. Diagonal blocks carry no pairs: their
 // sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take work items from a
 // host-built list: (block, pair range, flags), blocks with many pairs split into several items
 // whose partial sums are added atomically (kRcsAtomic; the output is zero-filled first) and only
@@ -419,7 +420,7 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   const int row = lane & 15, kg = lane >> 4;
   const bool live = row < ND;
   const int rowoff = (live ? row : 0) * 3;
-  d4 acc0 = {}, acc1 = {}, acc2 = {};
+  d4 acc = {};
   // A batch is kRcsBatch pairs = 3 kRcsBatch K entries = kRcsBatch * 3 / 4 MFMAs; its indices are
   // wave-uniform scalar loads, its Z values 2 per MFMA and lane, all issued before the MFMAs (the
   // loop is latency-bound: one round trip for the indices, one for Z, per batch). K entry 4h + kg
@@ -453,13 +454,10 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
     // interleaves them with the MFMAs four at a time, a round trip per four loads)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int h = 0; h < NH; h += 3) {
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h + 1], bv[h + 1], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h + 2], bv[h + 2], acc2, 0, 0, 0);
-    }
+    for (int h = 0; h < NH; ++h)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc, 0, 0, 0);
   }
-  const d4 sum = acc0 + acc1 + acc2;
+  const d4 sum = acc;
   const int j = row;
   if (j >= ND) return;
 #pragma unroll
