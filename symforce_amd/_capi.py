@@ -716,3 +716,28 @@ class HipCholesky:
         out = np.zeros(self.n)
         check(hip_lib().symhip_cholesky_d(self.h, ptr(out, C.c_double)))
         return out
+
+
+def compute_full_covariance(problem: "HipProblem", hessian_values):
+    """Optimizer::ComputeFullCovariance (opt/optimizer.tcc:202-206) through
+    LevenbergMarquardtSolver::ComputeCovariance (opt/levenberg_marquardt_solver.tcc:345-356): the
+    undamped lower Hessian of a linearization (values in the problem's CSC order) is factored by
+    the device LDL^T and solved against the identity; the result is the dense N x N inverse.
+    Raises, like the reference's assert, when the factorization fails."""
+    cp, ri = problem.hessian_structure()
+    chol = HipCholesky(problem.N, cp, ri)
+    chol.factorize(hessian_values)
+    cov = chol.solve(np.eye(problem.N))
+    if not np.isfinite(cov).all():
+        raise RuntimeError("Internal Error: damped hessian factorization failed")
+    return cov
+
+
+def compute_all_covariances(problem: "HipProblem", hessian_values):
+    """Optimizer::ComputeAllCovariances (opt/optimizer.tcc:113-121): the full covariance split by
+    key as SplitCovariancesByKey does (opt/internal/covariance_utils.h:172-192), the diagonal
+    block at each optimized key's tangent offset. Returns {key index in the problem: block}."""
+    cov = compute_full_covariance(problem, hessian_values)
+    keys, offsets = problem.state_keys()
+    ends = np.append(offsets[1:], problem.N)
+    return {int(k): cov[o:e, o:e].copy() for k, o, e in zip(keys, offsets, ends)}
