@@ -11,7 +11,8 @@ Prints one JSON line (value = LM iterations/s of the timed FP64 run) with:
   time_to_cost  wall time until the best cost is within 1e-6 relative of the FP64 oracle's final
                 cost (tests/golden/ba1k_oracle.json) for the plain and the fast forms
   roofline      the dominant kernel's achieved bandwidth (or FLOP rate) from HIP events
-  cpu_baseline  the CPU FP64 oracle timed on this host for two LM iterations of the same problem
+  cpu_baseline  the CPU FP64 oracle timed on this host for one LM iteration of the same problem
+                (about 70 s on one core; the one-time symbolic analysis is included)
 """
 from __future__ import annotations
 
