@@ -64,8 +64,9 @@ def run_gpu(args):
     opt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0), opts)
     t_setup = time.time() - t0
     info = opt.solver_info()
-    # warmup (same start point, same trajectory prefix)
-    opt.optimize_resident(num_iterations=args.warmup)
+    # warmup (same start point, same trajectory prefix); --warmup 0 skips it
+    if args.warmup > 0:
+        opt.optimize_resident(num_iterations=args.warmup)
     # timed: exactly K LM iterations, early exit disabled
     capi.hip_lib()  # ensure loaded
     t0 = time.perf_counter()
@@ -95,7 +96,8 @@ def run_gpu(args):
     if not args.no_fast:
         fopt = capi.HipOptimizer(prob, lm_params(args.warmup + args.steps, early_exit=0.0),
                                  fast_options(args, switch=False))
-        fopt.optimize_resident(num_iterations=args.warmup)
+        if args.warmup > 0:
+            fopt.optimize_resident(num_iterations=args.warmup)
         t0 = time.perf_counter()
         fres = fopt.optimize_resident(num_iterations=args.steps)
         fast_ms = 1e3 * (time.perf_counter() - t0) / (len(fres["iterations"]) - 1)
