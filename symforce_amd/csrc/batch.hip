@@ -245,6 +245,8 @@ __global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimi
   // Reset: the start Values go into the New block (block 1)
   if (live)
     for (int k = 0; k < 7; ++k) v.vals[(TP + g) * 7 + k] = v.cvals[v.pose_soff[g] + k];
+  // the first linearization reads the next lane's start pose (odometry), written just above
+  __syncthreads();
   int init = 0, nw = 1, best = 0, fre = 2, iteration = -1, best_index = 0;
   int status = 0, reason = 0, linearized = 0, nrec = 0;
   bool have_max = false, done = false;

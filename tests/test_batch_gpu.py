@@ -74,3 +74,20 @@ def test_unsupported_problems_are_rejected():
         capi.HipBatch([capi.gen_pose_smoothing(10, 42)])
     with pytest.raises(RuntimeError, match="1..64"):
         capi.HipBatch([capi.gen_localization(70, 10, 0.3, 1)])
+
+
+@pytest.mark.parametrize("lam", [capi.LAMBDA_STATIC, capi.LAMBDA_DYNAMIC])
+def test_full_batch_of_4096_problems_matches_oracle(lam):
+    """The benchmark's batched workload at its full size: 4096 localization problems of 5-50
+    poses and 5-40 landmarks in one launch, every problem's trajectory held to the oracle."""
+    probs = capi.gen_localization_batch(4096, seed=42)
+    params = capi.default_params(lambda_update_type=lam)
+    res = capi.HipBatch(probs, params).solve()
+    assert len(res["problems"]) == 4096
+    sizes = [len(p.values) for p in probs]
+    assert min(sizes) < 300 < max(sizes)  # the uneven sizes the launch is meant for
+    for prob, got in zip(probs, res["problems"]):
+        ora = ow.OracleProblem(prob).optimize(params)
+        assert_same_trajectory(got, ora, rtol=1e-8)
+        np.testing.assert_allclose(got["values"], ora["values"], rtol=0,
+                                   atol=1e-8 * max(1.0, np.abs(ora["values"]).max()))
