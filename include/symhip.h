@@ -74,6 +74,51 @@ int symhip_batch_solve(symhip_batch_t batch, double* values_out,
                        symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
                        double* solve_ms, double* total_ms);
 
+/* Allocation-free variant of symhip_batch_create + symhip_batch_solve. No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+   serving loops, harnesses that forbid hipMalloc). The caller packs the batch on the host with
+   symhip_batch_pack_sizes / symhip_batch_pack (no device memory), copies the tables into device
+   buffers it owns, and calls symhip_batch_solve_noalloc with device pointers for the tables and
+   the outputs. The kernel's working state (3 state blocks of poses, their linearizations, the
+   running maximum of the diagonal, clock stamps: 216 doubles per pose) lives in one file-scope
+   device array per GPU sized for SYMHIP_BATCH_NOALLOC_MAX_POSES poses in total (28.6 MB, held
+   by every GPU the library runs on until the process ends); larger batches are rejected with an
+   error. The call allocates nothing and launches on `stream` (a hipStream_t; NULL is the null
+   stream) without synchronizing. Calls that overlap on one GPU share that array and race:
+   serialize them. */
+#define SYMHIP_BATCH_NOALLOC_MAX_POSES 16384
+/* out[0] = problems, out[1] = total optimized poses, out[2] = matching factors,
+   out[3] = Values storage (doubles) over all problems. Validates like symhip_batch_create. */
+int symhip_batch_pack_sizes(const symhip_problem_desc* descs, int32_t n_problems,
+                            const symhip_optimizer_params* params, int64_t* out4);
+/* Host tables, sized from symhip_batch_pack_sizes: pose_begin (problems + 1), pose_soff (poses),
+   match_ptr (poses + 1), match_args (3 * matching factors), odo_args (3 * poses), values
+   (storage: the initial Values of every problem, concatenated in problem order). */
+int symhip_batch_pack(const symhip_problem_desc* descs, int32_t n_problems,
+                      const symhip_optimizer_params* params, int32_t* pose_begin,
+                      int32_t* pose_soff, int32_t* match_ptr, int32_t* match_args,
+                      int32_t* odo_args, double* values);
+typedef struct symhip_batch_tables {
+  int32_t n_problems;
+  int64_t total_poses;
+  int64_t storage_size;
+  /* device pointers to the packed tables (match_args may be NULL without matching factors) */
+  const int32_t* pose_begin;
+  const int32_t* pose_soff;
+  const int32_t* match_ptr;
+  const int32_t* match_args;
+  const int32_t* odo_args;
+  const double* values;
+} symhip_batch_tables;
+/* values_out (storage_size doubles), stats_out (n_problems records) and iters_out
+   (n_problems * (params.iterations + 1) records, or NULL) are device pointers; the results are
+   those of symhip_batch_solve on the same problems. */
+int symhip_batch_solve_noalloc(const symhip_batch_tables* tables,
+                               const symhip_optimizer_params* params, int32_t precision,
+                               double epsilon, double* values_out,
+                               symhip_optimization_stats* stats_out,
+                               symhip_iteration_stats* iters_out, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Optimizer: sym::Optimizer<double> + LevenbergMarquardtSolver with the linearization, linear
  * solve, damping, retraction and accept/lambda logic on the device (opt/optimizer.tcc:21-89,

@@ -334,6 +334,52 @@ def hip_lib():
     return _hip_lib
 
 
+class BatchTables(C.Structure):
+    """symhip_batch_tables: device pointers of a packed batch (symhip_batch_solve_noalloc)."""
+    _fields_ = [
+        ("n_problems", C.c_int32),
+        ("total_poses", C.c_int64),
+        ("storage_size", C.c_int64),
+        ("pose_begin", C.c_void_p),
+        ("pose_soff", C.c_void_p),
+        ("match_ptr", C.c_void_p),
+        ("match_args", C.c_void_p),
+        ("odo_args", C.c_void_p),
+        ("values", C.c_void_p),
+    ]
+
+
+BATCH_NOALLOC_MAX_POSES = 16384
+
+
+def batch_pack(problems, params: OptimizerParams | None = None):
+    """Host tables of a batch in the batched kernel's layout (symhip_batch_pack); no device
+    memory is touched."""
+    params = params or default_params()
+    n = len(problems)
+    descs = (ProblemDesc * n)(*[p.desc() for p in problems])
+    sizes = np.zeros(4, np.int64)
+    check(hip_lib().symhip_batch_pack_sizes(descs, n, C.byref(params), ptr(sizes, C.c_int64)))
+    _, poses, matches, storage = (int(x) for x in sizes)
+    out = dict(pose_begin=np.zeros(n + 1, np.int32), pose_soff=np.zeros(poses, np.int32),
+               match_ptr=np.zeros(poses + 1, np.int32),
+               match_args=np.zeros(max(1, 3 * matches), np.int32),
+               odo_args=np.zeros(3 * poses, np.int32), values=np.zeros(storage))
+    check(hip_lib().symhip_batch_pack(
+        descs, n, C.byref(params), *[ptr(out[k], C.c_int32) for k in
+                                      ("pose_begin", "pose_soff", "match_ptr", "match_args",
+                                       "odo_args")], ptr(out["values"], C.c_double)))
+    out.update(n_problems=n, total_poses=poses, n_matches=matches, storage_size=storage)
+    return out
+
+
+def batch_solve_noalloc(tables: BatchTables, params, precision, epsilon, values_out, stats_out,
+                        iters_out=None, stream=None):
+    """symhip_batch_solve_noalloc on device pointers (ints); launches without synchronizing."""
+    check(hip_lib().symhip_batch_solve_noalloc(C.byref(tables), C.byref(params), precision,
+                                               epsilon, values_out, stats_out, iters_out, stream))
+
+
 class SolverOptions(C.Structure):
     _fields_ = [
         ("linear_solver", C.c_int32),
@@ -389,6 +435,12 @@ _EXTRA_SIGNATURES: list = [
     ("symhip_batch_storage_size", [_vp], C.c_int64),
     ("symhip_batch_solve", [_vp, F64P, C.POINTER(OptimizationStats), C.POINTER(IterationStats),
                             F64P, F64P], C.c_int),
+    ("symhip_batch_pack_sizes", [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(OptimizerParams),
+                                 I64P], C.c_int),
+    ("symhip_batch_pack", [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(OptimizerParams), I32P,
+                           I32P, I32P, I32P, I32P, F64P], C.c_int),
+    ("symhip_batch_solve_noalloc", [C.POINTER(BatchTables), C.POINTER(OptimizerParams), C.c_int32,
+                                    C.c_double, _vp, _vp, _vp, _vp], C.c_int),
     ("symhip_optimizer_iteration_times", [_vp, F64P, C.c_int32], C.c_int32),
     ("symhip_optimizer_enable_timing", [_vp, C.c_int32], C.c_int),
     ("symhip_optimizer_phase_times", [_vp, F64P, I64P], C.c_int),

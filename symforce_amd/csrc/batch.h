@@ -6,8 +6,28 @@
 #include <vector>
 
 #include "./common.h"
+#include "../../include/symhip.h"
 
 namespace symhip {
+
+// Problem tables of a batch in the kernel's layout, built on the host (no device memory).
+struct PackedBatch {
+  std::vector<int32_t> pose_begin;  // [n_problems + 1] first global pose of each problem
+  std::vector<int32_t> pose_soff;   // [total_poses] offset of each pose's storage in cvals
+  std::vector<int32_t> match_ptr;   // [total_poses + 1] matching factors of each pose
+  std::vector<int32_t> match_args;  // 3 per match: landmark, measurement, sigma offsets
+  std::vector<int32_t> odo_args;    // 3 per pose: measurement, sigmas, epsilon (-1: last pose)
+  std::vector<double> cvals;        // every problem's Values storage, concatenated
+};
+PackedBatch pack_batch(const symhip_problem_desc* descs, int32_t n_problems,
+                       const symhip_optimizer_params& params);
+
+// symhip_batch_solve_noalloc: the batched LM kernel on caller-provided device tables, with its
+// working state in a file-scope device array (no allocation). Launches on `stream`.
+void batch_solve_noalloc(const symhip_batch_tables& t, const symhip_optimizer_params& p,
+                         int32_t precision, double epsilon, double* values_out,
+                         symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
+                         hipStream_t stream);
 
 class BatchLocalization {
  public:
@@ -39,7 +59,6 @@ class BatchLocalization {
   hipGraphExec_t graph_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   DBuf<int32_t> pose_begin_, pose_soff_, match_ptr_, match_args_, odo_args_;
-  DBuf<int64_t> storage_begin_;
   DBuf<double> cvals_, vout_, vals_, lin_, maxd_, err_;
   DBuf<symhip_iteration_stats> recs_;
   DBuf<symhip_optimization_stats> stats_;

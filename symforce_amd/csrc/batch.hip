@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "./factor_traits.h"
@@ -253,7 +254,8 @@ __global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimi
   double err[3] = {0, 0, 0};
   double lambda = p.initial_lambda, nu = p.dynamic_lambda_update_beta;
   auto record = [&](const symhip_iteration_stats& r) {
-    if (lane == 0 && nrec < v.rec_stride) v.recs[(int64_t)prob * v.rec_stride + nrec] = r;
+    if (lane == 0 && v.recs && nrec < v.rec_stride)
+      v.recs[(int64_t)prob * v.rec_stride + nrec] = r;
     ++nrec;
   };
   auto relin = [&](int b) {
@@ -368,16 +370,14 @@ __global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimi
 
 }  // namespace
 
-BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n_problems,
-                                     const symhip_optimizer_params& params, bool compute_f32,
-                                     double epsilon)
-    : n_problems_(n_problems), p_(params), f32_(compute_f32), eps_(epsilon) {
+PackedBatch pack_batch(const symhip_problem_desc* descs, int32_t n_problems,
+                       const symhip_optimizer_params& params) {
   SYMHIP_CHECK(descs && n_problems > 0, "batch: no problems");
   SYMHIP_CHECK(params.iterations > 0, "batch: iterations must be positive");
   SYMHIP_CHECK(!params.enable_bold_updates, "batch: bold updates are not supported");
-  std::vector<int32_t> pose_begin{0}, pose_soff, match_ptr{0}, match_args, odo_args;
-  std::vector<int64_t> storage_begin;
-  std::vector<double> cvals;
+  PackedBatch pk;
+  pk.pose_begin.push_back(0);
+  pk.match_ptr.push_back(0);
   for (int32_t pi = 0; pi < n_problems; ++pi) {
     HostProblem hp;
     build_host_problem(&descs[pi], hp);
@@ -389,10 +389,9 @@ BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n
                    "batch: optimized keys must be Pose3");
       pos_of[hp.state_keys[i]] = i;
     }
-    const int64_t base = (int64_t)cvals.size();
-    storage_begin.push_back(base);
-    cvals.insert(cvals.end(), hp.values.begin(), hp.values.end());
-    SYMHIP_CHECK(cvals.size() < (size_t)INT32_MAX, "batch: storage too large");
+    const int64_t base = (int64_t)pk.cvals.size();
+    pk.cvals.insert(pk.cvals.end(), hp.values.begin(), hp.values.end());
+    SYMHIP_CHECK(pk.cvals.size() < (size_t)INT32_MAX, "batch: storage too large");
     auto off = [&](int32_t key) { return (int32_t)(base + hp.storage_offset[key]); };
     auto constant = [&](int32_t key) {
       SYMHIP_CHECK(pos_of[key] < 0, "batch: factor constants must not be optimized");
@@ -419,24 +418,33 @@ BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n
       }
     }
     for (int32_t i = 0; i < n; ++i) {
-      pose_soff.push_back(off(hp.state_keys[i]));
-      match_args.insert(match_args.end(), matches[i].begin(), matches[i].end());
-      match_ptr.push_back((int32_t)(match_args.size() / 3));
-      odo_args.insert(odo_args.end(), odo.begin() + 3 * i, odo.begin() + 3 * i + 3);
+      pk.pose_soff.push_back(off(hp.state_keys[i]));
+      pk.match_args.insert(pk.match_args.end(), matches[i].begin(), matches[i].end());
+      pk.match_ptr.push_back((int32_t)(pk.match_args.size() / 3));
+      pk.odo_args.insert(pk.odo_args.end(), odo.begin() + 3 * i, odo.begin() + 3 * i + 3);
     }
-    pose_begin.push_back(pose_begin.back() + n);
+    pk.pose_begin.push_back(pk.pose_begin.back() + n);
   }
+  return pk;
+}
+
+BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n_problems,
+                                     const symhip_optimizer_params& params, bool compute_f32,
+                                     double epsilon)
+    : n_problems_(n_problems), p_(params), f32_(compute_f32), eps_(epsilon) {
+  PackedBatch pk = pack_batch(descs, n_problems, params);
+  std::vector<int32_t>& pose_begin = pk.pose_begin;
+  std::vector<double>& cvals = pk.cvals;
   total_poses_ = pose_begin.back();
   cvals_host_size_ = cvals.size();
   HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&ev0_));
   HIP_OK(hipEventCreate(&ev1_));
   pose_begin_.upload(pose_begin, stream_);
-  pose_soff_.upload(pose_soff, stream_);
-  match_ptr_.upload(match_ptr, stream_);
-  match_args_.upload(match_args.empty() ? std::vector<int32_t>{0} : match_args, stream_);
-  odo_args_.upload(odo_args, stream_);
-  storage_begin_.upload(storage_begin, stream_);
+  pose_soff_.upload(pk.pose_soff, stream_);
+  match_ptr_.upload(pk.match_ptr, stream_);
+  match_args_.upload(pk.match_args.empty() ? std::vector<int32_t>{0} : pk.match_args, stream_);
+  odo_args_.upload(pk.odo_args, stream_);
   cvals_.upload(cvals, stream_);
   vout_.upload(cvals, stream_);
   vals_.resize((size_t)3 * total_poses_ * 7);
@@ -502,6 +510,85 @@ void BatchLocalization::solve(double* values_out, symhip_optimization_stats* sta
     for (int32_t i = 0; i < n_problems_; ++i)
       solve_ms[i] = (double)(ticks[2 * i + 1] - ticks[2 * i]) / (double)khz;
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free solve: the kernel's working state (three state blocks of poses and their
+// linearizations, the running maximum of the diagonal and the per-problem clock stamps) lives in
+// a file-scope device array instead of buffers allocated per batch. Every call on a device shares
+// that device's array, so calls that overlap on one device race. Nothing in it has to start at
+// zero: each launch writes every entry it later reads (state block 1 from the inputs, the
+// linearizations before they are read, the maximum diagonal on its first damping).
+namespace {
+
+constexpr int64_t kPerPoseScratch = 3 * 7 + 3 * kLin + 6;  // state blocks, linearizations, max diag
+constexpr int64_t kScratchDoubles =
+    SYMHIP_BATCH_NOALLOC_MAX_POSES * kPerPoseScratch + 2 * SYMHIP_BATCH_NOALLOC_MAX_POSES;
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of the current device's copy of `scratch`, looked up once per device.
+double* device_scratch() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  SYMHIP_CHECK(dev >= 0 && dev < kMaxDevices, "batch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SYMHIP_CHECK(p != nullptr, "batch: scratch symbol has no device address");
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<double*>(g_scratch_ptr[dev]);
+}
+
+__global__ void copy_storage_kernel(int64_t n, const double* __restrict__ src,
+                                    double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+void batch_solve_noalloc(const symhip_batch_tables& t, const symhip_optimizer_params& p,
+                         int32_t precision, double epsilon, double* values_out,
+                         symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
+                         hipStream_t stream) {
+  SYMHIP_CHECK(t.n_problems > 0 && t.total_poses >= t.n_problems && t.storage_size > 0,
+               "batch_noalloc: empty or inconsistent tables");
+  SYMHIP_CHECK(t.total_poses <= SYMHIP_BATCH_NOALLOC_MAX_POSES,
+               "batch_noalloc: more poses than the scratch array holds "
+               "(SYMHIP_BATCH_NOALLOC_MAX_POSES)");
+  SYMHIP_CHECK(t.pose_begin && t.pose_soff && t.match_ptr && t.odo_args && t.values &&
+                   values_out && stats_out,
+               "batch_noalloc: null table or output pointer");
+  SYMHIP_CHECK(p.iterations > 0, "batch: iterations must be positive");
+  SYMHIP_CHECK(!p.enable_bold_updates, "batch: bold updates are not supported");
+  SYMHIP_CHECK(precision == SYMHIP_PRECISION_F64 || precision == SYMHIP_PRECISION_F32_JAC,
+               "batch_noalloc: unknown precision");
+  const int64_t TP = t.total_poses;
+  double* base = device_scratch();
+  double* vals = base;
+  double* lin = vals + 3 * TP * 7;
+  double* maxd = lin + 3 * TP * kLin;
+  int64_t* ticks = reinterpret_cast<int64_t*>(maxd + TP * 6);
+  // the output storage starts as the inputs; the kernel writes each problem's best poses into it
+  hipLaunchKernelGGL(copy_storage_kernel, dim3((unsigned)((t.storage_size + 255) / 256)),
+                     dim3(256), 0, stream, t.storage_size, t.values, values_out);
+  BatchView v{TP,          t.pose_begin, t.pose_soff, t.match_ptr,        t.match_args,
+              t.odo_args,  t.values,     vals,        lin,                maxd,
+              values_out,  iters_out,    p.iterations + 1, stats_out,     ticks};
+  if (precision == SYMHIP_PRECISION_F32_JAC)
+    hipLaunchKernelGGL(batch_lm_kernel<float>, dim3(t.n_problems), dim3(64), 0, stream, v, p,
+                       epsilon);
+  else
+    hipLaunchKernelGGL(batch_lm_kernel<double>, dim3(t.n_problems), dim3(64), 0, stream, v, p,
+                       epsilon);
+  HIP_OK(hipGetLastError());
 }
 
 }  // namespace symhip

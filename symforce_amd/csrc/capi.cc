@@ -414,6 +414,48 @@ int64_t symhip_batch_storage_size(symhip_batch_t batch) {
   return batch ? batch->b->storage_total() : -1;
 }
 
+int symhip_batch_pack_sizes(const symhip_problem_desc* descs, int32_t n_problems,
+                            const symhip_optimizer_params* params, int64_t* out4) {
+  return guarded([&] {
+    SYMHIP_CHECK(params && out4, "invalid arguments");
+    const symhip::PackedBatch pk = symhip::pack_batch(descs, n_problems, *params);
+    out4[0] = n_problems;
+    out4[1] = (int64_t)pk.pose_soff.size();
+    out4[2] = (int64_t)pk.match_args.size() / 3;
+    out4[3] = (int64_t)pk.cvals.size();
+  });
+}
+
+int symhip_batch_pack(const symhip_problem_desc* descs, int32_t n_problems,
+                      const symhip_optimizer_params* params, int32_t* pose_begin,
+                      int32_t* pose_soff, int32_t* match_ptr, int32_t* match_args,
+                      int32_t* odo_args, double* values) {
+  return guarded([&] {
+    SYMHIP_CHECK(params && pose_begin && pose_soff && match_ptr && odo_args && values,
+                 "invalid arguments");
+    const symhip::PackedBatch pk = symhip::pack_batch(descs, n_problems, *params);
+    SYMHIP_CHECK(match_args || pk.match_args.empty(), "invalid arguments: match_args");
+    std::copy(pk.pose_begin.begin(), pk.pose_begin.end(), pose_begin);
+    std::copy(pk.pose_soff.begin(), pk.pose_soff.end(), pose_soff);
+    std::copy(pk.match_ptr.begin(), pk.match_ptr.end(), match_ptr);
+    std::copy(pk.match_args.begin(), pk.match_args.end(), match_args);
+    std::copy(pk.odo_args.begin(), pk.odo_args.end(), odo_args);
+    std::copy(pk.cvals.begin(), pk.cvals.end(), values);
+  });
+}
+
+int symhip_batch_solve_noalloc(const symhip_batch_tables* tables,
+                               const symhip_optimizer_params* params, int32_t precision,
+                               double epsilon, double* values_out,
+                               symhip_optimization_stats* stats_out,
+                               symhip_iteration_stats* iters_out, void* stream) {
+  return guarded([&] {
+    SYMHIP_CHECK(tables && params, "invalid arguments");
+    symhip::batch_solve_noalloc(*tables, *params, precision, epsilon, values_out, stats_out,
+                                iters_out, static_cast<hipStream_t>(stream));
+  });
+}
+
 int symhip_batch_solve(symhip_batch_t batch, double* values_out,
                        symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
                        double* solve_ms, double* total_ms) {
