@@ -10,9 +10,14 @@ HBM before the timed region; the host reads back one iteration record per step.
 Prints one JSON line (value = LM iterations/s of the timed FP64 run) with:
   time_to_cost  wall time until the best cost is within 1e-6 relative of the FP64 oracle's final
                 cost (tests/golden/ba1k_oracle.json) for the plain and the fast forms
-  roofline      the dominant kernel's achieved bandwidth (or FLOP rate) from HIP events
-  cpu_baseline  the CPU FP64 oracle timed on this host for one LM iteration of the same problem
-                (about 70 s on one core; the one-time symbolic analysis is included)
+  roofline      the dominant work, the supernodal LDL^T factorization phase (per level a
+                diagonal/panel kernel and an update kernel; the phase is the largest share of an
+                iteration): FP64 flops from the symbolic analysis over the phase's HIP-event
+                time; the reduced-camera-system assembly kernel is reported beside it
+  cpu_baseline  the CPU FP64 oracle timed on this host's cores: 3 steady LM iterations of the
+                same problem after one untimed iteration (setup and the one-time symbolic
+                analysis excluded), the reduced camera system ordered by nested dissection as the
+                reference orders it with METIS; also the -ffast-math build of the same sources
 """
 from __future__ import annotations
 
@@ -37,16 +42,21 @@ ORACLE_FIXTURE = REPO / "tests" / "golden" / "ba1k_oracle.json"
 TRAFFIC_PROFILE = REPO / "profiles" / "ba1k_plain_traffic.txt"
 
 
-def measured_traffic(kernel_prefix):
-    """HBM bytes per launch of a kernel from the committed PMC summary (rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counters on a 512 MiB copy
-    kernel); None when the summary is absent."""
+def traffic_table():
+    """Per-kernel rows of the committed PMC summary (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    passes, each scaled by the same counters on a 512 MiB copy kernel): {kernel: (dispatches,
+    HBM MB per launch)}; empty when the summary is absent."""
+    rows = {}
     if not TRAFFIC_PROFILE.exists():
-        return None
+        return rows
     for line in TRAFFIC_PROFILE.read_text().splitlines():
-        if line.startswith(kernel_prefix):
-            return float(line.split()[-1]) * 1e6
-    return None
+        parts = line.split()
+        if len(parts) >= 5 and not line.startswith(("#", "kernel", "calibration")):
+            try:
+                rows[" ".join(parts[:-4])] = (int(parts[-4]), float(parts[-1]))
+            except ValueError:
+                continue
+    return rows
 
 
 def lm_params(iterations, early_exit=1e-6):
@@ -160,33 +170,66 @@ def speedup(plain, fast):
 
 
 def roofline(r):
-    """The largest single kernel launch of an LM iteration is the reduced-camera-system assembly
-    (one launch; its phase timer brackets exactly that kernel with HIP events on the optimizer's
-    stream). It reduces every off-diagonal S block over its observation pairs on the FP64
-    matrix cores (K = 3 per pair, packed densely: 4 pairs per 3 MFMAs): the useful work is
-    81 entries x 3 FMAs x 2 = 486 flops per pair, and the unique HBM bytes are the pair list, Z
-    once and the S blocks out (the diagonal blocks' own-observation sums are reduced per camera
-    in the reduced-rhs kernel). The supernodal LDL^T phase (one diagonal+panel launch and one
-    update launch per level) is reported beside it as a phase: flops from the symbolic analysis
-    over the phase time."""
+    """The dominant work of an LM iteration is the supernodal LDL^T factorization phase: per
+    level of the supernodal elimination tree one diagonal kernel (diag_trsm_kernel: each
+    supernode's diagonal block and its panel rows eliminated in one right-looking sweep, the
+    forward solve fused in) and one update kernel (update_kernel<64|32>: the trailing products
+    into ancestors on the FP64 matrix cores). Its algorithmic work is the symbolic analysis'
+    flop count (per supernode cols^3/3 + below x cols^2 + below^2 x cols), and the phase timer
+    brackets the phase with HIP events on the optimizer's stream, so achieved = flops / phase
+    time against the FP64 matrix peak. traffic: the HBM bytes of the phase's kernels per LM
+    iteration from the committed PMC summary.
+
+    The reduced-camera-system assembly (one launch per iteration) is reported beside it: 486
+    flops per off-diagonal observation pair on the FP64 matrix cores; unique bytes are the pair
+    list, Z once and the S blocks out."""
     info = r["info"]
     ph = r["phases"]
-    ms = ph["rcs_assemble"][0] / max(1, ph["rcs_assemble"][1])
-    n_obs = int(r["prob"].info["n_obs"])
-    flops = info["pairs"] * 486
-    bytes_ = info["pairs"] * 8 + n_obs * 27 * 8 + info["blocks"] * 81 * 8
-    achieved = flops / (ms * 1e-3) / 1e12
+    table = traffic_table()
+    iters_in_profile = table.get("void rcs_assemble_kernel<9, double, double>", (0, 0))[0]
     fac_ms = ph["factor"][0] / max(1, ph["factor"][1])
     fac_tf = info["flops"] / (fac_ms * 1e-3) / 1e12
-    return dict(kernel="rcs_assemble_kernel<9, double, double>", bound="mfma", achieved=achieved,
-                peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                traffic=measured_traffic("void rcs_assemble_kernel<9, double, double>"), ms_per_launch=ms,
-                algorithmic_flops=flops,
-                algorithmic_bytes=bytes_,
-                hbm_gbs=bytes_ / (ms * 1e-3) / 1e9,
-                ldlt_factor_phase={"ms": fac_ms, "flops": info["flops"], "tflops": fac_tf,
-                                   "frac_fp64": fac_tf / FP64_PEAK_TFLOPS,
-                                   "levels": info["levels"]})
+    fac_traffic = None
+    if iters_in_profile:
+        fac_traffic = sum(d * mb for k, (d, mb) in table.items()
+                          if k.startswith(("void diag_trsm_kernel", "void update_kernel"))
+                          ) * 1e6 / iters_in_profile
+    ms = ph["rcs_assemble"][0] / max(1, ph["rcs_assemble"][1])
+    n_obs = int(r["prob"].info["n_obs"])
+    rcs_flops = info["pairs"] * 486
+    rcs_bytes = info["pairs"] * 8 + n_obs * 27 * 8 + info["blocks"] * 81 * 8
+    rcs_tf = rcs_flops / (ms * 1e-3) / 1e12
+    rcs_traffic = table.get("void rcs_assemble_kernel<9, double, double>", (0, None))[1]
+    rcs_traffic = None if rcs_traffic is None else rcs_traffic * 1e6
+    return dict(
+        kernel=(f"supernodal LDL^T factorization phase: diag_trsm_kernel + update_kernel<64|32>"
+                f" per level, {info['levels']} levels"),
+        bound="mfma", achieved=fac_tf, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+        frac=fac_tf / FP64_PEAK_TFLOPS, traffic=fac_traffic, ms_per_launch=fac_ms,
+        algorithmic_flops=info["flops"],
+        flops_split={"diagonal_and_panels": info["diag_panel_flops"],
+                     "trailing_updates": info["update_flops"]},
+        levels=info["levels"], supernodes=info["supernodes"],
+        rcs_assemble={"kernel": "rcs_assemble_kernel<9, double, double>", "bound": "mfma",
+                      "achieved": rcs_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": rcs_tf / FP64_PEAK_TFLOPS, "ms_per_launch": ms,
+                      "algorithmic_flops": rcs_flops, "algorithmic_bytes": rcs_bytes,
+                      "traffic": rcs_traffic,
+                      "traffic_over_algorithmic": (None if rcs_traffic is None
+                                                   else rcs_traffic / rcs_bytes),
+                      "hbm_gbs": rcs_bytes / (ms * 1e-3) / 1e9})
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
 
 
 def batch_leg(n_problems=4096, cpu_sample=64):
@@ -209,29 +252,106 @@ def batch_leg(n_problems=4096, cpu_sample=64):
                      "converged": ok}
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
+    from concurrent.futures import ThreadPoolExecutor
+
+    def solve_cpu(prob):
+        ow.OracleProblem(prob).optimize(params)
+
+    model, cores = host_cpu()
     t0 = time.perf_counter()
     for prob in probs[:cpu_sample]:
-        ow.OracleProblem(prob).optimize(params)
-    dt = time.perf_counter() - t0
-    out["cpu_baseline"] = {"problems_per_s": cpu_sample / dt, "cores": 1, "kind": "port",
-                           "sample": f"first {cpu_sample} problems, FP64 oracle LM, sequential"}
+        solve_cpu(prob)
+    dt1 = time.perf_counter() - t0
+    # one optimizer per thread, as the reference allows (opt/optimizer.h:23); the oracle's calls
+    # release the interpreter lock, so the threads run on all cores
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(solve_cpu, probs))
+    dtn = time.perf_counter() - t0
+    out["cpu_baseline"] = {"problems_per_s": cpu_sample / dt1, "cores": 1, "kind": "port",
+                           "sample": f"first {cpu_sample} problems, FP64 oracle LM, sequential",
+                           "all_cores": {"problems_per_s": n_problems / dtn, "cores": cores,
+                                         "sample": f"all {n_problems} problems, one oracle "
+                                                   f"optimizer per thread"},
+                           "cpu": model}
     return out
 
 
-def cpu_baseline(prob, iters=1):
+def schur_elimination_order(prob):
+    """Elimination order of the reduced camera system for the CPU oracle: nested dissection of
+    the camera covisibility graph (the reference orders S with METIS, sparse_schur_solver.h:77),
+    each camera's 6 pose and 3 intrinsics columns kept together. State order is all poses, then
+    all intrinsics, then the points."""
+    ft, fp, fa = (np.asarray(x) for x in (prob.ftypes, prob.fptr, prob.fargs))
+    starts = fp[:-1][ft == capi.FACTOR_SNAVELY_HUBER]
+    cam_key, pt_key = fa[starts], fa[starts + 2]
+    ucam = np.unique(cam_key)
+    cam = np.searchsorted(ucam, cam_key)
+    o = np.lexsort((cam, pt_key))
+    pk, ck = pt_key[o], cam[o]
+    n = ucam.size
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for d in range(1, int(np.bincount(pk).max())):
+        same = pk[d:] == pk[:-d]
+        a, b = ck[:-d][same], ck[d:][same]
+        rows.append(np.maximum(a, b))
+        cols.append(np.minimum(a, b))
+    key = np.unique(np.concatenate(cols).astype(np.int64) * n + np.concatenate(rows))
+    c, r = key // n, key % n
+    colptr = np.concatenate([[0], np.cumsum(np.bincount(c, minlength=n))]).astype(np.int32)
+    order = capi.HipCholesky(n, colptr, r.astype(np.int32),
+                             ordering=capi.ORDERING_NESTED_DISSECTION).inv_permutation()
+    return np.concatenate([np.concatenate([6 * k + np.arange(6), 6 * n + 3 * k + np.arange(3)])
+                           for k in order]).astype(np.int32)
+
+
+def cpu_baseline(prob, iters=3):
+    """The CPU FP64 oracle (the restated reference LM + SparseSchurSolver) on this host: one
+    untimed LM iteration (it includes the one-time symbolic analysis), then `iters` timed ones,
+    single-threaded as the reference path is; then the same with the -ffast-math build."""
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
 
-    t0 = time.time()
-    op = ow.OracleProblem(prob)
-    t_create = time.time() - t0
-    secs, _ = ow.time_iterations(op, lm_params(iters, early_exit=0.0), iters, solver=ow.SCHUR,
-                                 c_dim=int(prob.info["c_dim"]))
-    per_iter = float(np.mean(secs))
+    order = schur_elimination_order(prob)
+    model, _ = host_cpu()
+    res = {}
+    for name, fast in (("plain", False), ("fast_math", True)):
+        t0 = time.time()
+        op = ow.OracleProblem(prob, fast_math=fast)
+        t_create = time.time() - t0
+        secs, _ = ow.time_iterations(op, lm_params(iters + 1, early_exit=0.0), iters + 1,
+                                     solver=ow.SCHUR, c_dim=int(prob.info["c_dim"]),
+                                     ordering=order)
+        del op
+        res[name] = dict(setup_s=t_create, first_iteration_s=float(secs[0]),
+                         s_per_iteration=float(np.mean(secs[1:])))
+    per_iter = res["plain"]["s_per_iteration"]
     return dict(value=1.0 / per_iter, unit="LM iters/s", cores=1, kind="port",
-                sample=f"CPU FP64 oracle (restated reference LM + SparseSchurSolver), same 1k-cam "
-                       f"problem, {iters} LM iteration(s) after {t_create:.1f} s setup: "
-                       f"{per_iter:.2f} s/iter (incl. the one-time symbolic analysis)")
+                sample=(f"CPU FP64 oracle (restated reference LM + SparseSchurSolver, S ordered "
+                        f"by nested dissection), same 1k-cam problem: {iters} steady LM "
+                        f"iterations after one untimed one (setup "
+                        f"{res['plain']['setup_s']:.1f} s and first iteration "
+                        f"{res['plain']['first_iteration_s']:.1f} s with the symbolic analysis, "
+                        f"both excluded): {per_iter:.2f} s/iter"),
+                cpu=model,
+                fast_math={"value": 1.0 / res["fast_math"]["s_per_iteration"],
+                           "s_per_iteration": res["fast_math"]["s_per_iteration"],
+                           "build": "-O3 -march=native -ffast-math"},
+                detail=res)
+
+
+def calibrate_copy(mib=512, launches=3):
+    """A device-to-device copy of known size under the PMC counters: bytes read = bytes written
+    = mib MiB per launch. Uses torch's copy kernel (torch is here only as the copy's harness)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_mib": mib, "launches": launches}))
 
 
 def main():
@@ -252,7 +372,13 @@ def main():
                     help="switch the fast form to the exact LDL^T below this LM reduction")
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the run-to-convergence (time-to-cost) leg, e.g. under a profiler")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 512 MiB device to device three times (a kernel of known HBM "
+                         "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the optimizer path runs on a single GPU "
               "(landmark sharding over several GPUs is designed but not built). Use --gpus 1.",

@@ -43,6 +43,11 @@ int64_t symhip_problem_col_starts(symhip_problem_t problem, int32_t* out);
 /* Optimized keys in state order and their tangent offsets (index_entry_t.offset). */
 int symhip_problem_state_keys(symhip_problem_t problem, int32_t* key_index, int32_t* tangent_offset);
 
+/* The registered factor type `code` (symhip_factor_type): out[0] = arguments, out[1] =
+   linearized arguments (the first ones; the rest are constants), out[2] = residual dimension,
+   out[3] = tangent dimension of the linearized arguments. */
+int symhip_factor_info(int32_t code, int32_t* out4);
+
 /* Linearizer::Relinearize (opt/linearizer.cc:54-120) at the given Values storage. Host buffers in
    and out (copies over PCIe; the optimizer keeps everything resident instead). Outputs may be
    NULL. hessian_values are in the reference CSC order. */
@@ -221,8 +226,10 @@ int symhip_optimizer_phase_times(symhip_optimizer_t opt, double* total_ms, int64
 
 /* Symbolic facts of the linear solver: out[0]=factorized dim, out[1]=supernodes,
    out[2]=panel doubles, out[3]=levels, out[4]=factor flops (rounded), out[5]=reduced-camera-
-   system blocks (Schur path, else 0), out[6]=landmark observation pairs, out[7]=observations. */
-int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out8);
+   system blocks (Schur path, else 0), out[6]=landmark observation pairs, out[7]=observations,
+   out[8]=flops of the diagonal blocks and panels (cols^3/3 + below-rows x cols^2 per
+   supernode), out[9]=flops of the trailing updates (below-rows^2 x cols). */
+int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out10);
 /* Iterative (PCG) linear solver statistics since creation: out[0]=solves, out[1]=total CG
    iterations, out[2]=fallbacks to the direct LDL^T, out[3]=most iterations in one solve.
    No reference counterpart (the reference solvers are direct). */
@@ -258,6 +265,47 @@ int symhip_optimizer_eval_preconditioner(symhip_optimizer_t opt, int32_t idx, do
                                          int32_t max_iters, int32_t learned, int32_t* iterations,
                                          double* step_err);
 
+/* debug_stats (optimizer_params_t.debug_stats): the vectors PopulateIterationStats records for
+   iteration record `index` of the last optimize call (levenberg_marquardt_solver.tcc:115-122,
+   :165-170): update (N; record 0, the iteration -1 record, has none and leaves it untouched),
+   the Values storage of the state block the record describes, and its residual (M). Any output
+   may be NULL. Fails when the optimizer was created without debug_stats. */
+int symhip_optimizer_debug_record(symhip_optimizer_t opt, int32_t index, double* update,
+                                  double* values, double* residual);
+/* OptimizationStats.linear_solver_ordering and .cholesky_factor_sparsity
+   (levenberg_marquardt_solver.tcc:205-212): the sparse Cholesky path's Permutation().indices()
+   (N entries, old -> new) and the strictly lower CSC pattern of its factor L (colptr N + 1,
+   rowind nnz). Returns nnz(L) (pass NULL outputs to query it), or a negative code; the Schur
+   path has neither (the reference's SparseSchurSolver exposes neither). */
+int64_t symhip_optimizer_factor_sparsity(symhip_optimizer_t opt, int32_t* ordering,
+                                         int32_t* colptr, int32_t* rowind);
+
+/* LevenbergMarquardtSolver::Iterate's accept / reject / lambda step
+   (levenberg_marquardt_solver.tcc:239-340), the same function the device optimizer runs, for
+   host loops over a user linearization (sym::LevenbergMarquardtSolver in include/sym/).
+   e_init / e_new / e_best: errors of the Init, New and Best state blocks; linear_delta:
+   update . (rhs - damping * update); last_norm2, update_norm2, last_dot_update: the bold-update
+   inputs |last|^2, |update|^2, last . update. */
+typedef struct symhip_lm_decision {
+  double lambda;
+  double nu;
+  symhip_iteration_stats record;
+  int32_t status;         /* symhip_status, 0 while iterating */
+  int32_t failure_reason; /* symhip_failure_reason */
+  int32_t accept;
+  int32_t set_best;
+} symhip_lm_decision;
+int symhip_lm_decide(double e_init, double e_new, double e_best, double linear_delta,
+                     double lambda, double nu, int32_t iteration,
+                     const symhip_optimizer_params* params, int32_t have_last_update,
+                     double last_norm2, double update_norm2, double last_dot_update,
+                     double epsilon, symhip_lm_decision* out);
+/* Values::Retract over the given keys (opt/values.cc:294-327): values (host storage) is moved
+   by delta (host tangent vector) at each key's storage and tangent offsets. */
+int symhip_retract(int32_t n_keys, const int32_t* types, const int32_t* storage_offsets,
+                   const int32_t* tangent_offsets, double* values, const double* delta,
+                   double epsilon);
+
 /* ---------------------------------------------------------------------------------------------
  * Linear solver adapter: SparseCholeskySolver (opt/sparse_cholesky/sparse_cholesky_solver.h:
  * 27-160) over a lower CSC matrix. node_ptr (n_nodes + 1 entries) groups consecutive scalars
@@ -283,6 +331,11 @@ int symhip_cholesky_etree(symhip_cholesky_t c, int32_t* parent, int32_t* counts)
 int symhip_cholesky_d(symhip_cholesky_t c, double* d);
 /* out[0]=n, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=factor flops. */
 int symhip_cholesky_info(symhip_cholesky_t c, int64_t* out5);
+/* SparseCholeskySolver::L() (sparse_cholesky_solver.h:146-148): the strictly lower unit factor of
+   the permuted matrix as CSC on the reference's pattern for this permutation (colptr n + 1,
+   rowind / values nnz; values need a factorization). Returns nnz; NULL outputs query it. */
+int64_t symhip_cholesky_factor(symhip_cholesky_t c, int32_t* colptr, int32_t* rowind,
+                               double* values);
 
 #ifdef __cplusplus
 }

@@ -478,11 +478,13 @@ struct Schur {
     for (int32_t col = 0; col < B; ++col)
       for (int32_t q = A.colptr[col]; q < A.colptr[col + 1]; ++q)
         if (A.rowind[q] >= B) et_rows[A.rowind[q] - B].push_back({col, q});
-    // S pattern = lower(B) U lower(E^T C^-1 E) structural
-    std::vector<Triplet> t;
+    // S pattern = lower(B) U lower(E^T C^-1 E) structural: marked in a dense B x B mask and
+    // read out column by column, which gives the sorted, duplicate-free CSC that setFromTriplets
+    // would (B is the small camera-side dimension; the dense position table below is B x B too)
+    std::vector<uint8_t> mask((size_t)B * B, 0);
     for (int32_t col = 0; col < B; ++col)
       for (int32_t q = A.colptr[col]; q < A.colptr[col + 1]; ++q)
-        if (A.rowind[q] < B) t.push_back({A.rowind[q], col, 0.0});
+        if (A.rowind[q] < B) mask[(size_t)col * B + A.rowind[q]] = 1;
     for (const CBlock& b : blocks) {
       std::vector<int32_t> cols;
       for (int32_t r = 0; r < b.dim; ++r)
@@ -490,9 +492,18 @@ struct Schur {
       std::sort(cols.begin(), cols.end());
       cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
       for (size_t x = 0; x < cols.size(); ++x)
-        for (size_t y = 0; y <= x; ++y) t.push_back({cols[x], cols[y], 0.0});
+        for (size_t y = 0; y <= x; ++y) mask[(size_t)cols[y] * B + cols[x]] = 1;
     }
-    S = csc_from_triplets(B, B, t);
+    S = Csc{};
+    S.rows = S.cols = B;
+    S.colptr.assign(B + 1, 0);
+    for (int32_t col = 0; col < B; ++col) {
+      const uint8_t* m = mask.data() + (size_t)col * B;
+      for (int32_t row = col; row < B; ++row)
+        if (m[row]) S.rowind.push_back(row);
+      S.colptr[col + 1] = (int32_t)S.rowind.size();
+    }
+    S.val.assign(S.rowind.size(), 0.0);
     s_pos.assign((int64_t)B * B, -1);
     for (int32_t col = 0; col < B; ++col)
       for (int32_t q = S.colptr[col]; q < S.colptr[col + 1]; ++q)

@@ -453,6 +453,11 @@ _EXTRA_SIGNATURES: list = [
     ("symhip_cholesky_etree", [_vp, I32P, I32P], C.c_int),
     ("symhip_cholesky_d", [_vp, F64P], C.c_int),
     ("symhip_cholesky_info", [_vp, I64P], C.c_int),
+    ("symhip_cholesky_factor", [_vp, I32P, I32P, F64P], C.c_int64),
+    ("symhip_optimizer_debug_record", [_vp, C.c_int32, F64P, F64P, F64P], C.c_int),
+    ("symhip_optimizer_factor_sparsity", [_vp, I32P, I32P, I32P], C.c_int64),
+    ("symhip_factor_info", [C.c_int32, I32P], C.c_int),
+    ("symhip_retract", [C.c_int32, I32P, I32P, I32P, F64P, F64P, C.c_double], C.c_int),
 ]
 
 
@@ -584,11 +589,12 @@ class HipOptimizer:
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(PHASE_NAMES)}
 
     def solver_info(self):
-        out = np.zeros(8, np.int64)
+        out = np.zeros(10, np.int64)
         check(hip_lib().symhip_optimizer_solver_info(self.h, ptr(out, C.c_int64)))
         return dict(n=int(out[0]), supernodes=int(out[1]), panel=int(out[2]),
                     levels=int(out[3]), flops=int(out[4]), blocks=int(out[5]),
-                    pairs=int(out[6]), obs=int(out[7]))
+                    pairs=int(out[6]), obs=int(out[7]), diag_panel_flops=int(out[8]),
+                    update_flops=int(out[9]))
 
     # ---- learned PCG preconditioner (Schur + use_pcg) ----
     def log_systems(self, max_systems):
@@ -744,6 +750,21 @@ class HipCholesky:
         c = np.zeros(self.n, np.int32)
         check(hip_lib().symhip_cholesky_etree(self.h, ptr(p, C.c_int32), ptr(c, C.c_int32)))
         return p, c
+
+    def factor_l(self):
+        """L() after a factorization: (colptr, rowind, values) of the strictly lower unit factor
+        of the permuted matrix on the reference's pattern."""
+        nnz = hip_lib().symhip_cholesky_factor(self.h, None, None, None)
+        if nnz < 0:
+            check(int(nnz))
+        cp = np.zeros(self.n + 1, np.int32)
+        ri = np.zeros(nnz, np.int32)
+        v = np.zeros(nnz)
+        got = hip_lib().symhip_cholesky_factor(self.h, ptr(cp, C.c_int32), ptr(ri, C.c_int32),
+                                                ptr(v, C.c_double))
+        if got < 0:
+            check(int(got))
+        return cp, ri, v
 
     def info(self):
         out = np.zeros(5, np.int64)

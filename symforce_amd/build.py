@@ -73,8 +73,8 @@ def build_cpp_tests():
     out = REPO / "tests" / "cpp" / "_build"
     out.mkdir(parents=True, exist_ok=True)
     for src in (REPO / "tests" / "cpp").glob("*.cc"):
-        _run(["g++", "-O2", "-std=c++17", "-I", str(REPO / "include"), str(src), "-o",
-              str(out / src.stem), "-L", str(OUT), "-lsymhip",
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(REPO / "include"), str(src), "-o",
+              str(out / src.stem), "-L", str(OUT), "-lsymhip", "-lsymhip_problems",
               "-Wl,-rpath,$ORIGIN/../../../symforce_amd/_build"])
 
 

@@ -10,6 +10,9 @@
 #include "./chol_numeric.h"
 #include "./linearize.h"
 #include "./batch.h"
+#include "./factor_traits.h"
+#include "./geo.h"
+#include "./lm_logic.h"
 #include "./optimizer.h"
 #include "./problem.h"
 
@@ -89,6 +92,19 @@ int symhip_problem_create(const symhip_problem_desc* desc, symhip_problem_t* out
     auto p = std::make_unique<symhip_problem>();
     symhip::build_host_problem(desc, p->host);
     *out = p.release();
+  });
+}
+
+int symhip_factor_info(int32_t code, int32_t* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(out, "invalid arguments");
+    symhip::dispatch_factor_type(code, [&]<int C>() {
+      using F = symhip::FT<C>;
+      out[0] = F::NA;
+      out[1] = F::NL;
+      out[2] = F::D;
+      out[3] = F::T;
+    });
   });
 }
 
@@ -233,6 +249,62 @@ int symhip_optimizer_phase_times(symhip_optimizer_t o, double* total_ms, int64_t
   });
 }
 
+int symhip_optimizer_debug_record(symhip_optimizer_t o, int32_t index, double* update,
+                                  double* values, double* residual) {
+  return guarded([&] {
+    SYMHIP_CHECK(o, "invalid arguments");
+    const auto& recs = o->opt->debug_records();
+    SYMHIP_CHECK(!recs.empty(), "no debug records: create the optimizer with debug_stats set");
+    SYMHIP_CHECK(index >= 0 && index < (int32_t)recs.size(), "debug record index out of range");
+    const auto& r = recs[index];
+    if (update) std::copy(r.update.begin(), r.update.end(), update);
+    if (values) std::copy(r.values.begin(), r.values.end(), values);
+    if (residual) std::copy(r.residual.begin(), r.residual.end(), residual);
+  });
+}
+
+int64_t symhip_optimizer_factor_sparsity(symhip_optimizer_t o, int32_t* ordering,
+                                         int32_t* colptr, int32_t* rowind) {
+  int64_t nnz = -1;
+  const int rc = guarded([&] {
+    SYMHIP_CHECK(o, "invalid arguments");
+    std::vector<int32_t> ord, lp, li;
+    o->opt->backend().factor_sparsity(ord, lp, li);
+    nnz = (int64_t)li.size();
+    if (ordering) std::copy(ord.begin(), ord.end(), ordering);
+    if (colptr) std::copy(lp.begin(), lp.end(), colptr);
+    if (rowind) std::copy(li.begin(), li.end(), rowind);
+  });
+  return rc == 0 ? nnz : rc;
+}
+
+int symhip_lm_decide(double e_init, double e_new, double e_best, double linear_delta,
+                     double lambda, double nu, int32_t iteration,
+                     const symhip_optimizer_params* params, int32_t have_last_update,
+                     double last_norm2, double update_norm2, double last_dot_update,
+                     double epsilon, symhip_lm_decision* out) {
+  return guarded([&] {
+    SYMHIP_CHECK(params && out, "invalid arguments");
+    const symhip::LmDecision d =
+        symhip::lm_decide(e_init, e_new, e_best, linear_delta, lambda, nu, iteration, *params,
+                          have_last_update, last_norm2, update_norm2, last_dot_update, epsilon);
+    *out = symhip_lm_decision{d.lambda, d.nu, d.rec, d.status, d.reason, d.accept, d.set_best};
+  });
+}
+
+int symhip_retract(int32_t n_keys, const int32_t* types, const int32_t* storage_offsets,
+                   const int32_t* tangent_offsets, double* values, const double* delta,
+                   double epsilon) {
+  return guarded([&] {
+    SYMHIP_CHECK(n_keys >= 0 && (n_keys == 0 || (types && storage_offsets && tangent_offsets &&
+                                                 values && delta)),
+                 "invalid arguments");
+    for (int32_t k = 0; k < n_keys; ++k)
+      symhip::retract_value(types[k], values + storage_offsets[k], delta + tangent_offsets[k],
+                            epsilon);
+  });
+}
+
 int symhip_optimizer_solver_info(symhip_optimizer_t o, int64_t* out) {
   return guarded([&] { o->opt->backend().info(out); });
 }
@@ -368,6 +440,25 @@ int symhip_cholesky_d(symhip_cholesky_t c, double* d) {
     c->ldlt.download_factor(panels, dd, c->stream);
     std::memcpy(d, dd.data(), sizeof(double) * dd.size());
   });
+}
+
+int64_t symhip_cholesky_factor(symhip_cholesky_t c, int32_t* colptr, int32_t* rowind,
+                               double* values) {
+  int64_t nnz = -1;
+  const int rc = guarded([&] {
+    SYMHIP_CHECK(c, "invalid arguments");
+    std::vector<int32_t> lp, li;
+    symhip::scalar_factor_pattern(c->sym.n, c->colptr.data(), c->rowind.data(), c->sym.perm, lp,
+                                  li);
+    nnz = (int64_t)li.size();
+    if (colptr) std::copy(lp.begin(), lp.end(), colptr);
+    if (rowind) std::copy(li.begin(), li.end(), rowind);
+    if (values) {
+      SYMHIP_CHECK(c->stream, "symhip_cholesky_factor: no factorization yet");
+      c->ldlt.factor_values(lp, li, values, c->stream);
+    }
+  });
+  return rc == 0 ? nnz : rc;
 }
 
 int symhip_cholesky_info(symhip_cholesky_t c, int64_t* out) {

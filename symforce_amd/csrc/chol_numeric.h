@@ -37,6 +37,10 @@ class SupernodalLDLT {
   const CholSymbolic& symbolic() const { return sym_; }
   // Downloads the panels (L21 below the diagonal blocks; the diagonal blocks keep A11) and D.
   void download_factor(std::vector<double>& panels, std::vector<double>& d, hipStream_t s) const;
+  // The factor's values on the reference's scalar pattern (scalar_factor_pattern of the same
+  // permutation): strictly lower unit L, from the factored diagonal blocks and the panels.
+  void factor_values(const std::vector<int32_t>& l_colptr, const std::vector<int32_t>& l_rowind,
+                     double* values, hipStream_t s) const;
   int64_t assembly_nnz() const { return (int64_t)asm_map_.size(); }
 
  private:

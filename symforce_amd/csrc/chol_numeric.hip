@@ -792,6 +792,36 @@ void SupernodalLDLT::launch_backward(double* x, hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
+void SupernodalLDLT::factor_values(const std::vector<int32_t>& l_colptr,
+                                   const std::vector<int32_t>& l_rowind, double* values,
+                                   hipStream_t s) const {
+  std::vector<double> panels(sym_.panel_size()), l11(l11_.size());
+  panel_.download(panels.data(), panels.size(), s);
+  l11_.download(l11.data(), l11.size(), s);
+  HIP_OK(hipStreamSynchronize(s));
+  int64_t l11_off = 0;
+  for (int32_t k = 0; k < sym_.n_snodes; ++k) {
+    const int32_t c0 = sym_.sn_col_begin[k], c1 = sym_.sn_col_begin[k + 1], n = c1 - c0;
+    const int32_t r0 = sym_.sn_rows_ptr[k], nrows = sym_.sn_rows_ptr[k + 1] - r0;
+    const int32_t* rows = sym_.sn_rows.data() + r0;
+    for (int32_t j = c0; j < c1; ++j) {
+      const int32_t jj = j - c0;
+      for (int32_t q = l_colptr[j]; q < l_colptr[j + 1]; ++q) {
+        const int32_t r = l_rowind[q];
+        if (r < c1) {
+          values[q] = l11[l11_off + (int64_t)jj * n + (r - c0)];
+        } else {
+          const int32_t* pos = std::lower_bound(rows + n, rows + nrows, r);
+          SYMHIP_CHECK(pos != rows + nrows && *pos == r,
+                       "factor pattern entry outside the supernodal panel");
+          values[q] = panels[sym_.sn_panel_off[k] + (int64_t)jj * nrows + (pos - rows)];
+        }
+      }
+    }
+    l11_off += (int64_t)n * n;
+  }
+}
+
 void SupernodalLDLT::download_factor(std::vector<double>& panels, std::vector<double>& d,
                                      hipStream_t s) const {
   panels.resize(sym_.panel_size());

@@ -262,6 +262,34 @@ void scalar_etree_counts(int32_t n, const int32_t* colptr, const int32_t* rowind
   }
 }
 
+void scalar_factor_pattern(int32_t n, const int32_t* colptr, const int32_t* rowind,
+                           const std::vector<int32_t>& perm, std::vector<int32_t>& l_colptr,
+                           std::vector<int32_t>& l_rowind) {
+  std::vector<int32_t> parent, counts;
+  scalar_etree_counts(n, colptr, rowind, perm, parent, counts);
+  l_colptr.assign(n + 1, 0);
+  for (int32_t j = 0; j < n; ++j) l_colptr[j + 1] = l_colptr[j] + counts[j];
+  l_rowind.assign(l_colptr[n], 0);
+  std::vector<std::vector<int32_t>> upper(n);
+  for (int32_t c = 0; c < n; ++c)
+    for (int32_t q = colptr[c]; q < colptr[c + 1]; ++q) {
+      const int32_t r = rowind[q];
+      const int32_t pr = perm.empty() ? r : perm[r], pc = perm.empty() ? c : perm[c];
+      if (pr != pc) upper[std::max(pr, pc)].push_back(std::min(pr, pc));
+    }
+  // row k of L is the union of the etree paths from the entries of column k of the upper
+  // pattern; visiting k in increasing order appends each column's rows sorted
+  std::vector<int32_t> visited(n, -1), fill(l_colptr.begin(), l_colptr.end() - 1);
+  for (int32_t k = 0; k < n; ++k) {
+    visited[k] = k;
+    for (int32_t i0 : upper[k])
+      for (int32_t i = i0; visited[i] != k; i = parent[i]) {
+        l_rowind[fill[i]++] = k;
+        visited[i] = k;
+      }
+  }
+}
+
 void chol_symbolic(int32_t n, const int32_t* colptr, const int32_t* rowind,
                    const std::vector<int32_t>& node_ptr, Ordering ordering,
                    const std::vector<int32_t>* given_node_order, int32_t max_cols,
@@ -413,6 +441,7 @@ void chol_symbolic(int32_t n, const int32_t* colptr, const int32_t* rowind,
     s.max_snode_rows = std::max(s.max_snode_rows, nrows);
     const double nc = ncols, mb = nrows - ncols;
     s.factor_flops += nc * nc * nc / 3.0 + mb * nc * nc + mb * mb * nc;
+    s.update_flops += mb * mb * nc;
     s.l_nnz += (int64_t)nrows * ncols - (int64_t)ncols * (ncols + 1) / 2;
   }
   s.sn_rows.resize(s.sn_rows_ptr[ns]);

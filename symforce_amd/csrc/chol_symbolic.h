@@ -44,6 +44,7 @@ struct CholSymbolic {
   int64_t panel_size() const { return sn_panel_off.empty() ? 0 : sn_panel_off.back(); }
   int64_t l_nnz = 0;       // nnz of strictly lower L implied by the supernodes (incl. padding)
   double factor_flops = 0; // flops of the numeric factorization (sum over supernodes)
+  double update_flops = 0; // of which the trailing updates of ancestors (below-rows^2 x cols)
   int32_t max_snode_cols = 0, max_snode_rows = 0;
 };
 
@@ -82,5 +83,11 @@ std::vector<int64_t> chol_assembly_map(const CholSymbolic& s, const int32_t* col
 void scalar_etree_counts(int32_t n, const int32_t* colptr, const int32_t* rowind,
                          const std::vector<int32_t>& perm, std::vector<int32_t>& parent,
                          std::vector<int32_t>& counts);
+
+// The reference's strictly lower L pattern (CSC, rows sorted) of P A P^T: what
+// SparseCholeskySolver::L() holds after a factorization (sparse_cholesky_solver.tcc:26-219).
+void scalar_factor_pattern(int32_t n, const int32_t* colptr, const int32_t* rowind,
+                           const std::vector<int32_t>& perm, std::vector<int32_t>& l_colptr,
+                           std::vector<int32_t>& l_rowind);
 
 }  // namespace symhip

@@ -15,6 +15,8 @@
 
 #include <cmath>
 
+#include "../../include/symhip_types.h"
+
 #define SYMHIP_HD __host__ __device__ __forceinline__
 
 namespace symhip {
@@ -161,6 +163,29 @@ SYMHIP_HD void rot3_retract_inplace(double* q, const double* w, double eps) {
   q[1] = r.y;
   q[2] = r.z;
   q[3] = r.w;
+}
+
+// Values::Retract of one key (opt/values.cc:294-327, gen/cpp/sym/ops/pose3/lie_group_ops.cc:69-102):
+// Pose3 rotates on the right and translates additively, Rot3 rotates on the right, every vector
+// and matrix type adds its tangent (storage) entries.
+SYMHIP_HD void retract_value(int type, double* v, const double* d, double eps) {
+  if (type == SYMHIP_TYPE_POSE3) {
+    rot3_retract_inplace(v, d, eps);
+    v[4] += d[3];
+    v[5] += d[4];
+    v[6] += d[5];
+  } else if (type == SYMHIP_TYPE_ROT3) {
+    rot3_retract_inplace(v, d, eps);
+  } else {
+    const int td = type == SYMHIP_TYPE_SCALAR    ? 1
+                   : type == SYMHIP_TYPE_VECTOR2  ? 2
+                   : type == SYMHIP_TYPE_VECTOR3  ? 3
+                   : type == SYMHIP_TYPE_VECTOR4  ? 4
+                   : type == SYMHIP_TYPE_VECTOR6  ? 6
+                   : type == SYMHIP_TYPE_MATRIX33 ? 9
+                                                  : 36;
+    for (int k = 0; k < td; ++k) v[k] += d[k];
+  }
 }
 
 // Robust whitening by the norm (symforce/opt/noise_models.py:68-90):

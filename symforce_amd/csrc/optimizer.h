@@ -49,11 +49,19 @@ class SystemBackend {
   virtual int32_t M() const = 0;
   virtual int64_t nnz() const = 0;
   // out[0]=factorized dim, out[1]=supernodes, out[2]=panel doubles, out[3]=levels, out[4]=flops,
-  // out[5]=RCS blocks, out[6]=observation pairs, out[7]=observations
-  virtual void info(int64_t* out8) const = 0;
+  // out[5]=RCS blocks, out[6]=observation pairs, out[7]=observations, out[8]=diagonal-block
+  // and panel flops, out[9]=trailing-update flops
+  virtual void info(int64_t* out10) const = 0;
   // Iterative linear solver statistics: out[0]=solves, out[1]=total iterations, out[2]=fallbacks
   // to the direct factorization, out[3]=most iterations in one solve (zeros for direct solvers).
   virtual void linear_stats(int64_t* out4) const { out4[0] = out4[1] = out4[2] = out4[3] = 0; }
+  // The linear solver's permutation (Permutation().indices(): old -> new) and the reference's
+  // strictly lower pattern of its factor L (debug_stats; sparse Cholesky path only, as the
+  // reference's SparseSchurSolver exposes neither).
+  virtual void factor_sparsity(std::vector<int32_t>& ordering, std::vector<int32_t>& l_colptr,
+                               std::vector<int32_t>& l_rowind) const {
+    throw Error("symhip: the factor sparsity is available on the sparse Cholesky path only");
+  }
   // Called at the start of every optimize run and after every LM iteration record.
   virtual void begin_run() {}
   virtual void observe(const symhip_iteration_stats& rec) { (void)rec; }
@@ -118,10 +126,19 @@ class DeviceOptimizer {
   double last_iteration_ms() const { return last_iter_ms_; }
   PhaseTimer& timer() { return timer_; }
   const std::vector<double>& iteration_times() const { return iter_times_; }
+  // debug_stats vectors of the last optimize run, one per iteration record (record 0 is the
+  // iteration -1 record and has no update): the update, the New block's Values storage and its
+  // residual (PopulateIterationStats, levenberg_marquardt_solver.tcc:115-122).
+  struct DebugRecord {
+    std::vector<double> update, values, residual;
+  };
+  const std::vector<DebugRecord>& debug_records() const { return debug_; }
 
  private:
   bool iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index, int& status,
                int& reason);
+  void record_debug(int block, bool with_update);
+  std::vector<DebugRecord> debug_;
   void relinearize(int b);
 
   HostProblem host_;

@@ -70,26 +70,7 @@ __global__ void retract_kernel(int32_t n, const int32_t* __restrict__ type,
                                double eps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double* v = values + soff[i];
-  const double* d = update + toff[i];
-  const int t = type[i];
-  if (t == SYMHIP_TYPE_POSE3) {
-    rot3_retract_inplace(v, d, eps);
-    v[4] += d[3];
-    v[5] += d[4];
-    v[6] += d[5];
-  } else if (t == SYMHIP_TYPE_ROT3) {
-    rot3_retract_inplace(v, d, eps);
-  } else {
-    const int td = t == SYMHIP_TYPE_SCALAR ? 1
-                   : t == SYMHIP_TYPE_VECTOR2 ? 2
-                   : t == SYMHIP_TYPE_VECTOR3 ? 3
-                   : t == SYMHIP_TYPE_VECTOR4 ? 4
-                   : t == SYMHIP_TYPE_VECTOR6 ? 6
-                   : t == SYMHIP_TYPE_MATRIX33 ? 9
-                                                : 36;
-    for (int k = 0; k < td; ++k) v[k] += d[k];
-  }
+  retract_value(type[i], values + soff[i], update + toff[i], eps);
 }
 
 // sum_i u_i (rhs_i - damping_i u_i)  (Linearization::LinearDeltaError, linearization.h:62-67)
@@ -173,6 +154,8 @@ class CscBackend final : public SystemBackend {
     x_.resize(std::max(1, p.N));
     partials_.resize(kReduceBlocks);
     colptr_.upload(p.colptr, s);
+    colptr_host_ = p.colptr;
+    rowind_host_ = p.rowind;
     // Symbolic analysis (the reference's AnalyzeSparsityPattern, done once): nodes are the
     // optimized keys.
     std::vector<int32_t> node_ptr{0};
@@ -218,6 +201,13 @@ class CscBackend final : public SystemBackend {
   void hessian_csc(int b, double* out, hipStream_t s) override {
     hval_[b].download(out, nnz_, s);
   }
+  void factor_sparsity(std::vector<int32_t>& ordering, std::vector<int32_t>& l_colptr,
+                       std::vector<int32_t>& l_rowind) const override {
+    const CholSymbolic& y = chol_.symbolic();
+    ordering = y.perm;
+    scalar_factor_pattern(N_, colptr_host_.data(), rowind_host_.data(), y.perm, l_colptr,
+                          l_rowind);
+  }
   int32_t N() const override { return N_; }
   int32_t M() const override { return M_; }
   int64_t nnz() const override { return nnz_; }
@@ -229,6 +219,8 @@ class CscBackend final : public SystemBackend {
     out[2] = y.panel_size();
     out[3] = y.n_levels;
     out[4] = (int64_t)y.factor_flops;
+    out[8] = (int64_t)(y.factor_flops - y.update_flops);
+    out[9] = (int64_t)y.update_flops;
     out[5] = out[6] = out[7] = 0;
   }
 
@@ -238,6 +230,7 @@ class CscBackend final : public SystemBackend {
   DBuf<double> residual_[3], hval_[3], rhs_[3];
   DBuf<double> undamped_, damping_, max_diag_, x_, partials_;
   DBuf<int32_t> colptr_;
+  std::vector<int32_t> colptr_host_, rowind_host_;
   int32_t N_ = 0, M_ = 0;
   int64_t nnz_ = 0;
 };
@@ -351,6 +344,7 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
   hipLaunchKernelGGL(lm_reset_kernel, dim3(1), dim3(1), 0, stream_, dstate_, p_.initial_lambda,
                      p_.dynamic_lambda_update_beta);
   backend_->begin_run();
+  debug_.clear();
   iters.clear();
   iter_times_.clear();
   int32_t best_index = 0;
@@ -377,6 +371,22 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
   stats.failure_reason = reason;
 }
 
+void DeviceOptimizer::record_debug(int block, bool with_update) {
+  DebugRecord r;
+  r.values.resize(host_.storage_total);
+  r.residual.resize(host_.M);
+  values_[block].download(r.values.data(), r.values.size(), stream_);
+  if (host_.M)
+    HIP_OK(hipMemcpyAsync(r.residual.data(), backend_->residual(block), sizeof(double) * host_.M,
+                          hipMemcpyDeviceToHost, stream_));
+  if (with_update) {
+    r.update.resize(host_.N);
+    update_.download(r.update.data(), r.update.size(), stream_);
+  }
+  HIP_OK(hipStreamSynchronize(stream_));
+  debug_.push_back(std::move(r));
+}
+
 bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index,
                               int& status, int& reason) {
   std::swap(init_, new_);  // state_.Step()
@@ -394,6 +404,7 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
     HIP_OK(hipMemcpyAsync(hstate_, dstate_, sizeof(LmDevState), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     iters.push_back(hstate_->rec);
+    if (p_.debug_stats) record_debug(init_, false);
     if (hstate_->status == SYMHIP_STATUS_FAILED) {
       status = hstate_->status;
       reason = hstate_->reason;
@@ -442,6 +453,7 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   timer_.collect();
   HIP_OK(hipGetLastError());
   iters.push_back(hstate_->rec);
+  if (p_.debug_stats) record_debug(new_, true);
   backend_->observe(hstate_->rec);
   if (!hstate_->accept) {
     std::swap(new_, init_);  // SwapNewAndInit

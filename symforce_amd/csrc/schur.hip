@@ -652,6 +652,8 @@ class SchurBackend final : public SystemBackend {
     out[2] = y.panel_size();
     out[3] = y.n_levels;
     out[4] = (int64_t)y.factor_flops;
+    out[8] = (int64_t)(y.factor_flops - y.update_flops);
+    out[9] = (int64_t)y.update_flops;
     out[5] = n_blocks_;
     out[6] = n_pairs_;
     out[7] = n_obs_;

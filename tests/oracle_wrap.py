@@ -11,17 +11,20 @@ from symforce_amd import _capi as capi
 
 REPO = Path(__file__).resolve().parent.parent
 ORACLE_LIB = REPO / "oracle" / "_build" / "liboracle.so"
+# the same sources built with -ffast-math (the reference's recommended benchmark flags,
+# symforce/benchmarks/README.md:15); used only to time the CPU baseline
+ORACLE_FASTMATH_LIB = REPO / "oracle" / "_build" / "liboracle_fastmath.so"
 CHOLESKY, SCHUR = 0, 1
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not ORACLE_LIB.exists():
-            raise RuntimeError(f"{ORACLE_LIB} not built (make -C oracle)")
-        L = C.CDLL(str(ORACLE_LIB))
+def lib(fast_math=False):
+    path = ORACLE_FASTMATH_LIB if fast_math else ORACLE_LIB
+    if path not in _libs:
+        if not path.exists():
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        L = C.CDLL(str(path))
         vp = C.c_void_p
         L.oracle_problem_create.argtypes = [C.POINTER(capi.ProblemDesc), C.c_int]
         L.oracle_problem_create.restype = vp
@@ -60,8 +63,8 @@ def lib():
                                             C.POINTER(capi.OptimizationStats), capi.F64P]
         L.oracle_linearize_factor.argtypes = [C.c_int32, C.POINTER(capi.F64P), capi.F64P,
                                               capi.F64P, capi.F64P, capi.F64P]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a, t):
@@ -69,44 +72,45 @@ def _p(a, t):
 
 
 class OracleProblem:
-    def __init__(self, prob: capi.Problem, include_jacobians=False):
+    def __init__(self, prob: capi.Problem, include_jacobians=False, fast_math=False):
         self.prob = prob
         self._desc = prob.desc()
-        self.h = lib().oracle_problem_create(C.byref(self._desc), int(include_jacobians))
+        self.L = lib(fast_math)
+        self.h = self.L.oracle_problem_create(C.byref(self._desc), int(include_jacobians))
         if not self.h:
-            raise RuntimeError(lib().oracle_last_error().decode())
+            raise RuntimeError(self.L.oracle_last_error().decode())
         dims = np.zeros(4, np.int64)
-        lib().oracle_dims(self.h, _p(dims, C.c_int64))
+        self.L.oracle_dims(self.h, _p(dims, C.c_int64))
         self.N, self.M, self.nnz, self.jnnz = (int(x) for x in dims)
         self.include_jacobians = include_jacobians
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().oracle_problem_destroy(self.h)
+            self.L.oracle_problem_destroy(self.h)
             self.h = None
 
     def hessian_structure(self):
         cp = np.zeros(self.N + 1, np.int32)
         ri = np.zeros(self.nnz, np.int32)
-        lib().oracle_hessian_structure(self.h, _p(cp, C.c_int32), _p(ri, C.c_int32))
+        self.L.oracle_hessian_structure(self.h, _p(cp, C.c_int32), _p(ri, C.c_int32))
         return cp, ri
 
     def jacobian_structure(self):
         cp = np.zeros(self.N + 1, np.int32)
         ri = np.zeros(self.jnnz, np.int32)
-        lib().oracle_jacobian_structure(self.h, _p(cp, C.c_int32), _p(ri, C.c_int32))
+        self.L.oracle_jacobian_structure(self.h, _p(cp, C.c_int32), _p(ri, C.c_int32))
         return cp, ri
 
     def state_keys(self):
         k = np.zeros(self.prob.n_keys, np.int32)
         o = np.zeros(self.prob.n_keys, np.int32)
-        n = lib().oracle_state_keys(self.h, _p(k, C.c_int32), _p(o, C.c_int32))
+        n = self.L.oracle_state_keys(self.h, _p(k, C.c_int32), _p(o, C.c_int32))
         return k[:n], o[:n]
 
     def hessian_col_starts(self):
-        n = lib().oracle_hessian_col_starts(self.h, None)
+        n = self.L.oracle_hessian_col_starts(self.h, None)
         out = np.zeros(n, np.int32)
-        lib().oracle_hessian_col_starts(self.h, _p(out, C.c_int32))
+        self.L.oracle_hessian_col_starts(self.h, _p(out, C.c_int32))
         return out
 
     def linearize(self, values=None):
@@ -115,10 +119,10 @@ class OracleProblem:
         h = np.zeros(self.nnz)
         b = np.zeros(self.N)
         j = np.zeros(self.jnnz) if self.include_jacobians else None
-        rc = lib().oracle_linearize(self.h, _p(v, C.c_double), _p(r, C.c_double),
+        rc = self.L.oracle_linearize(self.h, _p(v, C.c_double), _p(r, C.c_double),
                                     _p(h, C.c_double), _p(b, C.c_double), _p(j, C.c_double))
         if rc != 0:
-            raise RuntimeError(lib().oracle_last_error().decode())
+            raise RuntimeError(self.L.oracle_last_error().decode())
         return dict(residual=r, hessian_values=h, rhs=b, jacobian_values=j)
 
     def optimize(self, params=None, solver=CHOLESKY, ordering=None, c_dim=0, epsilon=None,
@@ -132,11 +136,11 @@ class OracleProblem:
         ordv = None if ordering is None else np.ascontiguousarray(ordering, np.int32)
         nmax = (params.iterations if num_iterations < 0 else num_iterations) + 1
         upd = np.zeros(nmax * self.N) if record_updates else None
-        rc = lib().oracle_optimize(self.h, C.byref(params), solver, _p(ordv, C.c_int32), c_dim,
+        rc = self.L.oracle_optimize(self.h, C.byref(params), solver, _p(ordv, C.c_int32), c_dim,
                                    eps, num_iterations, _p(vals, C.c_double), iters, maxit,
                                    C.byref(st), _p(upd, C.c_double))
         if rc != 0:
-            raise RuntimeError(lib().oracle_last_error().decode())
+            raise RuntimeError(self.L.oracle_last_error().decode())
         out = dict(values=vals, status=st.status, failure_reason=st.failure_reason,
                    best_index=st.best_index,
                    iterations=capi.iteration_records(iters, st.num_iterations))
@@ -145,18 +149,21 @@ class OracleProblem:
         return out
 
 
-def time_iterations(op: "OracleProblem", params, num_iterations, solver=CHOLESKY, c_dim=0):
-    """Runs num_iterations LM iterations and returns the seconds spent in each Iterate()."""
+def time_iterations(op: "OracleProblem", params, num_iterations, solver=CHOLESKY, c_dim=0,
+                    ordering=None):
+    """Runs num_iterations LM iterations and returns the seconds spent in each Iterate()
+    (ordering: elimination order of the factorized matrix, H or the Schur complement S)."""
     vals = op.prob.values.copy()
     iters = (capi.IterationStats * (num_iterations + 2))()
     st = capi.OptimizationStats()
     secs = np.zeros(num_iterations)
     eps = op.prob.info.get("epsilon", 2.220446049250313e-15)
-    rc = lib().oracle_optimize_timed(op.h, C.byref(params), solver, None, c_dim, eps,
-                                     num_iterations, _p(vals, C.c_double), iters,
-                                     num_iterations + 2, C.byref(st), _p(secs, C.c_double))
+    ordv = None if ordering is None else np.ascontiguousarray(ordering, np.int32)
+    rc = op.L.oracle_optimize_timed(op.h, C.byref(params), solver, _p(ordv, C.c_int32), c_dim,
+                                    eps, num_iterations, _p(vals, C.c_double), iters,
+                                    num_iterations + 2, C.byref(st), _p(secs, C.c_double))
     if rc != 0:
-        raise RuntimeError(lib().oracle_last_error().decode())
+        raise RuntimeError(op.L.oracle_last_error().decode())
     return secs, capi.iteration_records(iters, st.num_iterations)
 
 

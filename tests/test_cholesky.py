@@ -114,3 +114,10 @@ def test_solve_matches_oracle_and_dense(name, ordering):
     X = ch.solve(B)
     XO = oc.solve(B)
     assert np.linalg.norm(X - XO) <= rtol * np.linalg.norm(XO)
+    # L(): the reference's pattern for this permutation, bit for bit, and its values
+    lo = oc.factor()
+    lp, li, lv = ch.factor_l()
+    np.testing.assert_array_equal(lp, lo["colptr"])
+    np.testing.assert_array_equal(li, lo["rowind"])
+    np.testing.assert_allclose(lv, lo["values"], rtol=0,
+                               atol=rtol * max(1.0, np.abs(lo["values"]).max()))
