@@ -376,22 +376,24 @@ __global__ void __launch_bounds__(256) point_elim_kernel(int32_t n_obs,
 // feeds A(i = l % 16, k = l / 16) = X(i, 4h + k) and B(k, j = l % 16) = Y(j, 4h + k) (rows
 // i >= ND and K entries past the block's pairs are zero) and holds D(l / 16 + 4 r, l % 16) in
 // accumulator register r (the FP64 16x16x4 output interleaves rows across the four lane groups).
-// A batch's observation indices are wave-uniform scalar loads (the pair list is padded so a
-// batch reaching past the last block stays inside it). The kernel is latency-bound, so registers
-// are what count: four-pair batches into one accumulator (44 VGPRs, eight wavefronts per SIMD)
-// take 296 us at 1000 cameras, against 326 us for eight-pair batches into three accumulators
-// (100 VGPRs, five wavefronts) and 320 us for eight pairs into one. Diagonal blocks carry no pairs: their
-// sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves take work items from a
-// host-built list: (block, pair range, flags), blocks with many pairs split into several items
-// whose partial sums are added atomically (kRcsAtomic; the output is zero-filled first) and only
-// the first item of a block adds the B / damping / ZZ terms (kRcsFirst); block -1 = idle wave.
+// The pair list is padded so a batch reaching past the last block stays inside it. Diagonal
+// blocks carry no pairs: their sum_o Z_o Z_o^T (ZZ_a) comes from the per-camera reduction. Waves
+// take work items from a host-built list: (block, pair range, flags), blocks with many pairs
+// split into several items whose partial sums are added atomically (kRcsAtomic; the output is
+// zero-filled first) and only the first item of a block adds the B / damping / ZZ terms
+// (kRcsFirst); block -1 = idle wave. At 1000 cameras the launch moves ~5 TB/s of Z copies past
+// L2 (each Z block is read once per pair it is in, ~10 times, against ~2 for perfect reuse) with
+// the matrix cores a third busy: traffic, not a wave's latency, is its bound, and the item order
+// trades L2 reuse (row order) against the launch's tail (largest items first).
 // Output: into the LDL^T panels at (base + col * ld + row) or transposed, or (sfull != null)
 // into full symmetric block storage, block row-major, at f_lo[blk] and its mirror f_up[blk]
 // (transposed). Z is read as TW (FP32 in the FP32 form) and widened: the sums are FP64 on the
 // matrix cores either way; the full storage is written as TS.
 constexpr int kRcsChunk = 512;  // pairs per work item at most
 constexpr int kRcsBatch = 4;    // pairs per batch (a multiple of 4: 3 K entries per pair, 4 per MFMA)
-constexpr bool kRcsHeaviestFirst = true;  // order each XCD's items by size (else by camera row)
+constexpr bool kRcsHeaviestFirst = false;  // order each XCD's items by size (else by camera row)
+constexpr double kRcsTailFrac = 0.4;       // row order: share of the pairs dealt heaviest first
+constexpr int kRcsTailChunk = 256;         // row order: pairs per item at most in that tail
 constexpr int kRcsFirst = 1, kRcsAtomic = 2;
 template <int ND, typename TW, typename TS>
 __global__ void __launch_bounds__(256) rcs_assemble_kernel(
@@ -421,42 +423,127 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   const bool live = row < ND;
   const int rowoff = (live ? row : 0) * 3;
   d4 acc = {};
-  // A batch is kRcsBatch pairs = 3 kRcsBatch K entries = kRcsBatch * 3 / 4 MFMAs; its indices are
-  // wave-uniform scalar loads, its Z values 2 per MFMA and lane, all issued before the MFMAs (the
-  // loop is latency-bound: one round trip for the indices, one for Z, per batch). K entry 4h + kg
-  // belongs to pair p = (4h + kg) / 3 (column (4h + kg) % 3), which is pair plo = 4h / 3 or
-  // plo + 1: a select between two wave-uniform indices. Indices past the item's end come from the
-  // next pairs or the padding (valid observations), masked out by ok.
+  // A batch is kRcsBatch pairs = 2 kRcsBatch observations = 3 kRcsBatch K entries =
+  // kRcsBatch * 3 / 4 MFMAs. Its Z blocks (ZS elements = CPO 16-byte chunks each) are copied
+  // whole into the wavefront's LDS buffer by direct-to-LDS loads (global_load_lds_dwordx4: lane
+  // t of load u writes chunk s = 64 u + t at byte 16 s, so observation slot o = s / CPO lands at
+  // o * ZS elements): every byte of a fetched cache line is used, where per-lane 8-byte gathers
+  // in the MFMA layout touched ~10 lines per instruction for 288 useful bytes. Slot o of pair
+  // p = o / 2 is its x (o even) or y (o odd), i.e. pairs[2 q + o]. At step h lane l reads its
+  // operands from LDS: K = 4h + l / 16 is pair K / 3, column K % 3. Pairs past the item's end
+  // come from the next pairs or the padding (valid observations), masked by ok.
   constexpr int ZS = z_stride<ND, TW>(), NB = kRcsBatch, NH = NB * 3 / 4;
-  for (int q = q_begin; q < q_end; q += NB) {
-    int xs[NB], ys[NB];
+  constexpr int CPO = ZS * (int)sizeof(TW) / 16;
+  constexpr int SLOTS = 2 * NB * CPO, NI = (SLOTS + 63) / 64;
+  static_assert(ZS * (int)sizeof(TW) % 16 == 0, "Z blocks must be whole 16-byte chunks");
+  static_assert(NI <= 15, "vmcnt immediate");
+  __shared__ __attribute__((aligned(16))) char lds_z[4][NI * 1024];
+  const int w = threadIdx.x >> 6;
+  // Every lane of every copy is active (a masked copy would be a branch, and the wait counts
+  // below assume NI copies per batch): slots past the batch's chunks re-read chunk 0 into the
+  // unused tail of the buffer.
+  int slot_o[NI], slot_c[NI];
 #pragma unroll
-    for (int p = 0; p < NB; ++p) {
-      xs[p] = pairs[2 * (q + p)];
-      ys[p] = pairs[2 * (q + p) + 1];
+  for (int u = 0; u < NI; ++u) {
+    const int sl = 64 * u + lane;
+    slot_o[u] = sl < SLOTS ? sl / CPO : 0;
+    slot_c[u] = sl < SLOTS ? sl % CPO : 0;
+  }
+  // Each lane fetches the observation index of its own slots with vector loads, one batch
+  // ahead (after the batch's copy, so the wait for the copy leaves them in flight). The index
+  // loads are always issued (a batch past the item's end re-reads its first pairs) to keep the
+  // wait count fixed.
+  auto fetch_idx = [&](int qb, int (&ix)[NI]) {
+    const int qs = qb < q_end ? qb : q_begin;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) ix[u] = pairs[2 * qs + slot_o[u]];
+  };
+  auto issue = [&](const int (&ix)[NI], char* buf) {
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const char* src = reinterpret_cast<const char*>(Z + (int64_t)ix[u] * ZS) + slot_c[u] * 16;
+      __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(buf + u * 1024),
+                                       16, 0, 0);
     }
+  };
+  // per-lane operand offsets (elements) of step h inside a batch buffer
+  int off_x[NH], off_y[NH];
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const int K = 4 * h + kg;
+    const int p = K / 3, col = K - 3 * p;
+    off_x[h] = (2 * p) * ZS + rowoff + col;
+    off_y[h] = (2 * p + 1) * ZS + rowoff + col;
+  }
+  // The operand reads are written as asm blocks (three steps' A and B operands each) that also
+  // wait for them: the compiler would otherwise put a wait for every outstanding direct-to-LDS
+  // copy (the next batch's included) before the first read.
+  static_assert(NH % 3 == 0, "operand reads go in groups of three MFMA steps");
+  auto read3 = [&](uint32_t base, int g, double* av, double* bv) {
+    if constexpr (sizeof(TW) == 8) {
+      asm volatile(
+          "ds_read_b64 %0, %6\n\t"
+          "ds_read_b64 %1, %7\n\t"
+          "ds_read_b64 %2, %8\n\t"
+          "ds_read_b64 %3, %9\n\t"
+          "ds_read_b64 %4, %10\n\t"
+          "ds_read_b64 %5, %11\n\t"
+          "s_waitcnt lgkmcnt(0)"
+          : "=&v"(av[0]), "=&v"(av[1]), "=&v"(av[2]), "=&v"(bv[0]), "=&v"(bv[1]), "=&v"(bv[2])
+          : "v"(base + 8 * off_x[g]), "v"(base + 8 * off_x[g + 1]), "v"(base + 8 * off_x[g + 2]),
+            "v"(base + 8 * off_y[g]), "v"(base + 8 * off_y[g + 1]), "v"(base + 8 * off_y[g + 2])
+          : "memory");
+    } else {
+      float fa[3], fb[3];
+      asm volatile(
+          "ds_read_b32 %0, %6\n\t"
+          "ds_read_b32 %1, %7\n\t"
+          "ds_read_b32 %2, %8\n\t"
+          "ds_read_b32 %3, %9\n\t"
+          "ds_read_b32 %4, %10\n\t"
+          "ds_read_b32 %5, %11\n\t"
+          "s_waitcnt lgkmcnt(0)"
+          : "=&v"(fa[0]), "=&v"(fa[1]), "=&v"(fa[2]), "=&v"(fb[0]), "=&v"(fb[1]), "=&v"(fb[2])
+          : "v"(base + 4 * off_x[g]), "v"(base + 4 * off_x[g + 1]), "v"(base + 4 * off_x[g + 2]),
+            "v"(base + 4 * off_y[g]), "v"(base + 4 * off_y[g + 1]), "v"(base + 4 * off_y[g + 2])
+          : "memory");
+#pragma unroll
+      for (int h = 0; h < 3; ++h) {
+        av[h] = (double)fa[h];
+        bv[h] = (double)fb[h];
+      }
+    }
+  };
+  auto consume = [&](int q, const char* buf) {
+    const uint32_t base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const char*)buf;
     double av[NH], bv[NH];
 #pragma unroll
+    for (int g = 0; g < NH; g += 3) read3(base, g, av + g, bv + g);
+#pragma unroll
     for (int h = 0; h < NH; ++h) {
-      const int K = 4 * h + kg;
-      const int p = K / 3, col = K - 3 * p;
-      const int plo = (4 * h) / 3;
-      const bool hi = p > plo;
-      const int x = hi ? xs[plo + 1] : xs[plo];
-      const int y = hi ? ys[plo + 1] : ys[plo];
+      const int p = (4 * h + kg) / 3;
       const bool ok = live && q + p < q_end;
-      const double za = (double)Z[(int64_t)x * ZS + rowoff + col];
-      const double zb = (double)Z[(int64_t)y * ZS + rowoff + col];
-      av[h] = ok ? za : 0.0;
-      bv[h] = ok ? zb : 0.0;
+      av[h] = ok ? av[h] : 0.0;
+      bv[h] = ok ? bv[h] : 0.0;
     }
-    // all of the batch's loads go out before the first MFMA waits (left alone, the scheduler
-    // interleaves them with the MFMAs four at a time, a round trip per four loads)
-    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int h = 0; h < NH; ++h)
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h], bv[h], acc, 0, 0, 0);
+  };
+  // Per batch: copy batch k, fetch batch k + 1's indices, wait for the copy (the index loads
+  // issued after it may stay in flight: vector memory operations complete in order), read,
+  // multiply. (A second buffer, batch k + 1's copy overlapping batch k's reads and products,
+  // measured no faster: the launch is bound by the copies' HBM and L2 traffic, not by a wave's
+  // latency.)
+  int ix[NI];
+  fetch_idx(q_begin, ix);
+  for (int q = q_begin; q < q_end; q += NB) {
+    issue(ix, lds_z[w]);
+    fetch_idx(q + NB, ix);
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(NI) : "memory");
+    consume(q, lds_z[w]);
   }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the last index loads
   const d4 sum = acc;
   const int j = row;
   if (j >= ND) return;
@@ -985,10 +1072,10 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
   }
   // Assembly work list. Workgroups are dispatched round-robin over the XCDs (XCD = workgroup
   // index mod kXcds), each with its own L2: XCD x takes a contiguous range of camera rows holding
-  // about 1/kXcds of the pairs. Blocks with more than kRcsChunk pairs become several items, and
-  // each XCD's items run largest first, so the longest waves start early (row order keeps more of
-  // Z in L2, a quarter of the traffic, but measured slower: the kernel is latency-bound, not
-  // HBM-bound); -1 pads the XCDs to equal length.
+  // about 1/kXcds of the pairs; -1 pads the XCDs to equal length. At 1000 cameras, all items in
+  // row order cut the L2 misses to a third of the largest-first order's but end 30% later (a
+  // 512-pair item started last is the launch's tail); row order for the first 60% of the pairs
+  // and the rest largest first in items of at most 256 pairs is 6% faster than largest first.
   std::vector<int4> rcs_items;
   {
     std::vector<int64_t> before(n_nodes_ + 1, 0);  // pairs (+1 per block) in rows < a
@@ -997,20 +1084,32 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     std::vector<std::vector<int32_t>> per(kXcds);
     for (int32_t k = 0; k < n_blocks_; ++k)
       per[std::min<int64_t>(kXcds - 1, before[blk_a[k]] * kXcds / before[n_nodes_])].push_back(k);
-    // blocks with more than kRcsChunk pairs become several items (no wave carries a long tail)
+    // Each XCD's blocks in row order (consecutive rows share most of their Z blocks, which then
+    // stay in the XCD's L2), blocks with more than kRcsChunk pairs split into several items. The
+    // blocks of the last kRcsTailFrac of an XCD's pairs are split finer (kRcsTailChunk) and
+    // dealt heaviest first, so no long item starts at the end of the launch.
     std::vector<std::vector<int4>> its(kXcds);
-    for (int x = 0; x < kXcds; ++x)
+    auto add_items = [&](std::vector<int4>& out, int32_t k, int32_t chunk) {
+      const int32_t q0 = pcount[k], q1 = pcount[k + 1];
+      const int32_t nc = std::max<int32_t>(1, (q1 - q0 + chunk - 1) / chunk);
+      for (int32_t c = 0; c < nc; ++c)
+        out.push_back(make_int4(k, q0 + c * chunk, std::min(q1, q0 + (c + 1) * chunk),
+                                (c == 0 ? kRcsFirst : 0) | (nc > 1 ? kRcsAtomic : 0)));
+    };
+    for (int x = 0; x < kXcds; ++x) {
+      int64_t total = 0, seen = 0;
+      for (int32_t k : per[x]) total += pcount[k + 1] - pcount[k];
+      std::vector<int4> tail;
       for (int32_t k : per[x]) {
-        const int32_t q0 = pcount[k], q1 = pcount[k + 1];
-        const int32_t nc = std::max<int32_t>(1, (q1 - q0 + kRcsChunk - 1) / kRcsChunk);
-        for (int32_t c = 0; c < nc; ++c)
-          its[x].push_back(make_int4(k, q0 + c * kRcsChunk, std::min(q1, q0 + (c + 1) * kRcsChunk),
-                                     (c == 0 ? kRcsFirst : 0) | (nc > 1 ? kRcsAtomic : 0)));
+        const bool in_tail = kRcsHeaviestFirst || seen >= (int64_t)((1.0 - kRcsTailFrac) * total);
+        if (in_tail) add_items(tail, k, kRcsHeaviestFirst ? kRcsChunk : kRcsTailChunk);
+        else add_items(its[x], k, kRcsChunk);
+        seen += pcount[k + 1] - pcount[k];
       }
-    if (kRcsHeaviestFirst)
-      for (auto& v : its)
-        std::stable_sort(v.begin(), v.end(),
-                         [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+      std::stable_sort(tail.begin(), tail.end(),
+                       [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+      its[x].insert(its[x].end(), tail.begin(), tail.end());
+    }
     size_t len = 0;
     for (auto& v : its) len = std::max(len, v.size());
     len = (len + 3) / 4 * 4;
