@@ -283,12 +283,14 @@ using VectorNd = VectorN<double, N, T>;
 using Vector2d = VectorN<double, 2, SYMHIP_TYPE_VECTOR2>;
 using Vector3d = VectorN<double, 3, SYMHIP_TYPE_VECTOR3>;
 using Vector4d = VectorN<double, 4, SYMHIP_TYPE_VECTOR4>;
+using Vector5d = VectorN<double, 5, SYMHIP_TYPE_VECTOR5>;
 using Vector6d = VectorN<double, 6, SYMHIP_TYPE_VECTOR6>;
 using Matrix33d = VectorN<double, 9, SYMHIP_TYPE_MATRIX33>;   // column-major
 using Matrix66d = VectorN<double, 36, SYMHIP_TYPE_MATRIX66>;  // column-major
 using Vector2f = VectorN<float, 2, SYMHIP_TYPE_VECTOR2>;
 using Vector3f = VectorN<float, 3, SYMHIP_TYPE_VECTOR3>;
 using Vector4f = VectorN<float, 4, SYMHIP_TYPE_VECTOR4>;
+using Vector5f = VectorN<float, 5, SYMHIP_TYPE_VECTOR5>;
 using Vector6f = VectorN<float, 6, SYMHIP_TYPE_VECTOR6>;
 using Matrix33f = VectorN<float, 9, SYMHIP_TYPE_MATRIX33>;
 using Matrix66f = VectorN<float, 36, SYMHIP_TYPE_MATRIX66>;
@@ -301,6 +303,7 @@ inline int32_t StorageDim(int32_t type) {
     case SYMHIP_TYPE_VECTOR2: return 2;
     case SYMHIP_TYPE_VECTOR3: return 3;
     case SYMHIP_TYPE_VECTOR4: return 4;
+    case SYMHIP_TYPE_VECTOR5: return 5;
     case SYMHIP_TYPE_VECTOR6: return 6;
     case SYMHIP_TYPE_MATRIX33: return 9;
     case SYMHIP_TYPE_MATRIX66: return 36;
@@ -462,6 +465,9 @@ template <typename Scalar>
 constexpr FactorTag<SYMHIP_FACTOR_ODOMETRY> OdometryFactor{};
 template <typename Scalar>
 constexpr FactorTag<SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY> BetweenFactorPose3Cauchy{};
+// barron_factor(x, y, mu, eps) of test/symforce_gnc_codegen_test.py:20-29 (Vector5 x and y)
+template <typename Scalar>
+constexpr FactorTag<SYMHIP_FACTOR_BARRON_VECTOR5> BarronFactor{};
 
 // sym::Factor<S>: a registered factor body and the keys of its arguments. The optimized keys of
 // a registered factor are its linearized arguments (the first ones); a smaller set of optimized
@@ -1025,6 +1031,7 @@ class LevenbergMarquardtSolver {
       case SYMHIP_TYPE_VECTOR2: SetAs<Vector2d>(w, e.key, d); break;
       case SYMHIP_TYPE_VECTOR3: SetAs<Vector3d>(w, e.key, d); break;
       case SYMHIP_TYPE_VECTOR4: SetAs<Vector4d>(w, e.key, d); break;
+      case SYMHIP_TYPE_VECTOR5: SetAs<Vector5d>(w, e.key, d); break;
       case SYMHIP_TYPE_VECTOR6: SetAs<Vector6d>(w, e.key, d); break;
       case SYMHIP_TYPE_MATRIX33: SetAs<Matrix33d>(w, e.key, d); break;
       default: SetAs<Matrix66d>(w, e.key, d); break;
@@ -1108,51 +1115,7 @@ class Optimizer {
   virtual void Optimize(Values<Scalar>& values, int num_iterations,
                         bool populate_best_linearization, Stats& stats) {
     Initialize(values);
-    const optimizer_params_t& p = Params();
-    const int n = num_iterations < 0 ? p.iterations : num_iterations;
-    stats.Reset(n + 1);
-    std::vector<double> v(values.Data().begin(), values.Data().end());
-    std::vector<symhip_iteration_stats> recs(n + 2);
-    symhip_optimization_stats st{};
-    check(symhip_optimizer_optimize(opt_, v.data(), num_iterations, recs.data(),
-                                    (int32_t)recs.size(), &st));
-    for (size_t i = 0; i < values.Data().size(); ++i) values.Data()[i] = static_cast<Scalar>(v[i]);
-    const index_t all = values.CreateIndex();
-    for (int32_t i = 0; i < st.num_iterations; ++i) {
-      optimization_iteration_t it = FromRecord(recs[i]);
-      if (p.debug_stats) {
-        std::vector<double> upd(N_), vals(values.Data().size()), res(M_);
-        check(symhip_optimizer_debug_record(opt_, i, upd.data(), vals.data(), res.data()));
-        if (i > 0) it.update = ToVector(upd);
-        it.values.index = all;
-        it.values.data = vals;
-        it.residual = ToVector(res);
-      }
-      stats.iterations.push_back(std::move(it));
-    }
-    stats.best_index = st.best_index;
-    stats.status = st.status;
-    stats.failure_reason = st.failure_reason;
-    if (p.debug_stats && device_solver_ == SYMHIP_SOLVER_CHOLESKY) {
-      const int64_t nnz = symhip_optimizer_factor_sparsity(opt_, nullptr, nullptr, nullptr);
-      if (nnz < 0) check((int)nnz);
-      stats.linear_solver_ordering.resize(N_);
-      auto& L = stats.cholesky_factor_sparsity;
-      L.column_pointers.resize(N_ + 1);
-      L.row_indices.resize(nnz);
-      L.shape = {{N_, N_}};
-      const int64_t got = symhip_optimizer_factor_sparsity(
-          opt_, stats.linear_solver_ordering.data(), L.column_pointers.data(),
-          L.row_indices.data());
-      if (got < 0) check((int)got);
-    }
-    if (populate_best_linearization) {
-      Linearization<MatrixType> lin;
-      std::vector<double> r(M_), h(nnz_), b(N_);
-      check(symhip_optimizer_best_linearization(opt_, r.data(), h.data(), b.data()));
-      FillLinearization(lin, r, h, b);
-      stats.best_linearization = std::move(lin);
-    }
+    RunOnDevice(values, num_iterations, populate_best_linearization, stats, nullptr, -1);
   }
   void Optimize(Values<Scalar>& values, int num_iterations, Stats& stats) {
     Optimize(values, num_iterations, false, stats);
@@ -1230,6 +1193,61 @@ class Optimizer {
 
  protected:
   bool IsInitialized() const { return opt_ != nullptr; }
+
+  // The device LM run of Optimize, or (gnc != null) of GncOptimizer::Optimize with the Scalar
+  // key at mu_key_index of the Values as the convexity parameter.
+  void RunOnDevice(Values<Scalar>& values, int num_iterations, bool populate_best_linearization,
+                   Stats& stats, const symhip_gnc_params* gnc, int32_t mu_key_index) {
+    const optimizer_params_t& p = Params();
+    const int n = num_iterations < 0 ? p.iterations : num_iterations;
+    stats.Reset(n + 1);
+    std::vector<double> v(values.Data().begin(), values.Data().end());
+    std::vector<symhip_iteration_stats> recs(n + 2);
+    symhip_optimization_stats st{};
+    if (gnc)
+      check(symhip_optimizer_optimize_gnc(opt_, v.data(), num_iterations, gnc, mu_key_index,
+                                          recs.data(), (int32_t)recs.size(), &st));
+    else
+      check(symhip_optimizer_optimize(opt_, v.data(), num_iterations, recs.data(),
+                                      (int32_t)recs.size(), &st));
+    for (size_t i = 0; i < values.Data().size(); ++i) values.Data()[i] = static_cast<Scalar>(v[i]);
+    const index_t all = values.CreateIndex();
+    for (int32_t i = 0; i < st.num_iterations; ++i) {
+      optimization_iteration_t it = FromRecord(recs[i]);
+      if (p.debug_stats) {
+        std::vector<double> upd(N_), vals(values.Data().size()), res(M_);
+        check(symhip_optimizer_debug_record(opt_, i, upd.data(), vals.data(), res.data()));
+        if (i > 0) it.update = ToVector(upd);
+        it.values.index = all;
+        it.values.data = vals;
+        it.residual = ToVector(res);
+      }
+      stats.iterations.push_back(std::move(it));
+    }
+    stats.best_index = st.best_index;
+    stats.status = st.status;
+    stats.failure_reason = st.failure_reason;
+    if (p.debug_stats && device_solver_ == SYMHIP_SOLVER_CHOLESKY) {
+      const int64_t nnz = symhip_optimizer_factor_sparsity(opt_, nullptr, nullptr, nullptr);
+      if (nnz < 0) check((int)nnz);
+      stats.linear_solver_ordering.resize(N_);
+      auto& L = stats.cholesky_factor_sparsity;
+      L.column_pointers.resize(N_ + 1);
+      L.row_indices.resize(nnz);
+      L.shape = {{N_, N_}};
+      const int64_t got = symhip_optimizer_factor_sparsity(
+          opt_, stats.linear_solver_ordering.data(), L.column_pointers.data(),
+          L.row_indices.data());
+      if (got < 0) check((int)got);
+    }
+    if (populate_best_linearization) {
+      Linearization<MatrixType> lin;
+      std::vector<double> r(M_), h(nnz_), b(N_);
+      check(symhip_optimizer_best_linearization(opt_, r.data(), h.data(), b.data()));
+      FillLinearization(lin, r, h, b);
+      stats.best_linearization = std::move(lin);
+    }
+  }
 
   // Optimizer::Initialize (opt/optimizer.tcc:261-278): the problem structure is fixed after
   // the first call, as the reference Linearizer's is.
@@ -1317,6 +1335,46 @@ class Optimizer {
 
 using Optimizerd = Optimizer<double>;
 using Optimizerf = Optimizer<float>;
+
+// optimizer_gnc_params_t (lcmtypes/symforce.lcm:203-227)
+using optimizer_gnc_params_t = symhip_gnc_params;
+
+// GncOptimizer<BaseOptimizerType> (opt/gnc_optimizer.h:22-148): graduated non-convexity over an
+// Optimizer whose factors read the Scalar gnc_mu_key (0: convex, towards 1: robust). Optimize
+// sets mu to mu_initial, runs to convergence with early exit at gnc_update_min_reduction, then
+// repeatedly raises mu by mu_step (capped at mu_max, where the user's early exit returns),
+// relaxes the damping to its initial value and continues from the best Values, until a stage
+// ends without SUCCESS, mu is at its maximum, or the iteration budget (shared by all stages) is
+// spent. The stages run in the device optimizer (symhip_optimizer_optimize_gnc).
+template <typename BaseOptimizerType>
+class GncOptimizer : public BaseOptimizerType {
+ public:
+  using BaseOptimizer = BaseOptimizerType;
+  using Scalar = typename BaseOptimizer::Scalar;
+  using Stats = typename BaseOptimizer::Stats;
+
+  template <typename... OptimizerArgs>
+  GncOptimizer(const optimizer_params_t& optimizer_params,
+               const optimizer_gnc_params_t& gnc_params, const Key& gnc_mu_key,
+               OptimizerArgs&&... args)
+      : BaseOptimizer(optimizer_params, std::forward<OptimizerArgs>(args)...),
+        gnc_params_(gnc_params),
+        gnc_mu_key_(gnc_mu_key) {}
+  ~GncOptimizer() override = default;
+
+  using BaseOptimizerType::Optimize;
+  void Optimize(Values<Scalar>& values, int num_iterations, bool populate_best_linearization,
+                Stats& stats) override {
+    values.Set(gnc_mu_key_, static_cast<Scalar>(gnc_params_.mu_initial));
+    this->Initialize(values);
+    this->RunOnDevice(values, num_iterations, populate_best_linearization, stats, &gnc_params_,
+                      values.IndexOf(gnc_mu_key_));
+  }
+
+ private:
+  optimizer_gnc_params_t gnc_params_;
+  Key gnc_mu_key_;
+};
 
 // Optimize(params, factors, values) (opt/optimizer.h:334-340)
 template <typename Scalar, typename NonlinearSolverType = LevenbergMarquardtSolver<Scalar>>

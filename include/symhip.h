@@ -189,6 +189,17 @@ int symhip_optimizer_destroy(symhip_optimizer_t opt);
 int symhip_optimizer_optimize(symhip_optimizer_t opt, double* values, int32_t num_iterations,
                               symhip_iteration_stats* iters, int32_t max_iters,
                               symhip_optimization_stats* stats);
+/* GncOptimizer<Optimizer>::Optimize (opt/gnc_optimizer.h:49-118): Levenberg-Marquardt stages
+   to convergence, the Scalar key mu_key_index (an index into desc's keys; a constant read by the
+   factors, e.g. SYMHIP_FACTOR_BARRON_VECTOR5) set to gnc->mu_initial and raised by mu_step up to
+   mu_max after each stage that ends in SUCCESS. Stages before the last exit early at
+   gnc_update_min_reduction; damping is relaxed to the initial lambda between stages. The
+   records of all stages go to iters (one iteration -1 record, first); num_iterations (< 0:
+   params.iterations) bounds the total. values is updated with the last stage's best Values. */
+int symhip_optimizer_optimize_gnc(symhip_optimizer_t opt, double* values, int32_t num_iterations,
+                                  const symhip_gnc_params* gnc, int32_t mu_key_index,
+                                  symhip_iteration_stats* iters, int32_t max_iters,
+                                  symhip_optimization_stats* stats);
 /* Linearization of the best Values of the last optimize (stats.best_linearization). */
 int symhip_optimizer_best_linearization(symhip_optimizer_t opt, double* residual,
                                         double* hessian_values, double* rhs);

@@ -24,6 +24,7 @@ enum symhip_value_type {
   SYMHIP_TYPE_VECTOR2 = 12, /* storage 2, tangent 2 */
   SYMHIP_TYPE_VECTOR3 = 13, /* storage 3, tangent 3 */
   SYMHIP_TYPE_VECTOR4 = 14, /* storage 4, tangent 4 */
+  SYMHIP_TYPE_VECTOR5 = 15, /* storage 5, tangent 5 */
   SYMHIP_TYPE_VECTOR6 = 16, /* storage 6, tangent 6 */
   SYMHIP_TYPE_MATRIX33 = 37, /* storage 9 column-major, tangent 9 */
   SYMHIP_TYPE_MATRIX66 = 64, /* storage 36 column-major, tangent 36 */
@@ -61,10 +62,22 @@ enum symhip_factor_type {
   SYMHIP_FACTOR_ODOMETRY = 9,
   /* as BETWEEN_POSE3 plus cauchy_c Scalar before epsilon; whitened-norm Cauchy loss -> res 6 */
   SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY = 10,
+  /* x(*) V5, y V5, mu Scalar, epsilon Scalar -> res 5: per element the Barron loss of x - y with
+     alpha = 2 - 1/(1 - mu + epsilon), delta 1, unit information (the graduated non-convexity
+     factor of test/symforce_gnc_codegen_test.py:20-29, symforce/opt/noise_models.py:318-436) */
+  SYMHIP_FACTOR_BARRON_VECTOR5 = 11,
 };
 
 /* A problem as it crosses the boundary. Keys are given in Values append order; `values` is the
    flat storage of all keys in that order (reference Values::Data(), opt/values.h:30-324). */
+/* optimizer_gnc_params_t (lcmtypes/symforce.lcm:203-227): graduated non-convexity schedule. */
+typedef struct symhip_gnc_params {
+  double gnc_update_min_reduction; /* early exit threshold of every stage but the last */
+  double mu_initial;
+  double mu_step;                  /* mu <= 0 or mu_initial >= mu_max: one plain stage */
+  double mu_max;
+} symhip_gnc_params;
+
 typedef struct symhip_problem_desc {
   int32_t n_keys;
   const int8_t* key_letters;

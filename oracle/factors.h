@@ -19,6 +19,7 @@ inline int storage_dim(int type) {
     case SYMHIP_TYPE_VECTOR2: return 2;
     case SYMHIP_TYPE_VECTOR3: return 3;
     case SYMHIP_TYPE_VECTOR4: return 4;
+    case SYMHIP_TYPE_VECTOR5: return 5;
     case SYMHIP_TYPE_VECTOR6: return 6;
     case SYMHIP_TYPE_MATRIX33: return 9;
     case SYMHIP_TYPE_MATRIX66: return 36;
@@ -43,7 +44,8 @@ struct FactorSpec {
 inline FactorSpec factor_spec(int type) {
   const int P = SYMHIP_TYPE_POSE3, R = SYMHIP_TYPE_ROT3, S = SYMHIP_TYPE_SCALAR,
             V2 = SYMHIP_TYPE_VECTOR2, V3 = SYMHIP_TYPE_VECTOR3, V4 = SYMHIP_TYPE_VECTOR4,
-            V6 = SYMHIP_TYPE_VECTOR6, M3 = SYMHIP_TYPE_MATRIX33, M6 = SYMHIP_TYPE_MATRIX66;
+            V5 = SYMHIP_TYPE_VECTOR5, V6 = SYMHIP_TYPE_VECTOR6, M3 = SYMHIP_TYPE_MATRIX33,
+            M6 = SYMHIP_TYPE_MATRIX66;
   switch (type) {
     case SYMHIP_FACTOR_PRIOR_POSE3: return {4, {P, P, M6, S}, 0b1, 6};
     case SYMHIP_FACTOR_BETWEEN_POSE3: return {5, {P, P, P, M6, S}, 0b11, 6};
@@ -55,6 +57,7 @@ inline FactorSpec factor_spec(int type) {
     case SYMHIP_FACTOR_MATCHING: return {4, {P, V3, V3, S}, 0b1, 3};
     case SYMHIP_FACTOR_ODOMETRY: return {5, {P, P, P, V6, S}, 0b11, 6};
     case SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY: return {6, {P, P, P, M6, S, S}, 0b11, 6};
+    case SYMHIP_FACTOR_BARRON_VECTOR5: return {4, {V5, V5, S, S}, 0b1, 5};
   }
   throw std::runtime_error("oracle: unknown factor type");
 }
@@ -125,6 +128,26 @@ inline void whiten_cauchy(std::array<T, D>& r, double c, double eps) {
   const T n = sqrt(sq);
   const T s = c * sqrt(log(1.0 + sq / (c * c))) / n;
   for (int i = 0; i < D; ++i) r[i] = s * r[i];
+}
+
+// BarronNoiseModel (symforce/opt/noise_models.py:318-436) with delta 1, scalar_information 1,
+// x_epsilon = alpha_epsilon = eps, alpha from mu by compute_alpha_from_mu (:382-396), applied to
+// each element of x - y: barron_error (:398-416) and whiten_scalar (:418-436), the factor of
+// test/symforce_gnc_codegen_test.py:20-29.
+template <typename T, int D>
+inline std::array<T, D> res_barron(const std::array<T, D>& x, const double* y, double mu,
+                                   double eps) {
+  const double alpha = 2.0 - 1.0 / (1.0 - mu + eps);
+  const double b = std::fabs(alpha - 2.0) + eps;
+  const double sign_no_zero = alpha >= 0.0 ? 1.0 : -1.0;
+  const double d = alpha + sign_no_zero * eps;
+  std::array<T, D> r;
+  for (int i = 0; i < D; ++i) {
+    const T e = x[i] - y[i];
+    const T err = (b / d) * (pow((e * e) / b + 1.0, d / 2.0) - 1.0);
+    r[i] = sqrt(2.0 * err + eps) - std::sqrt(eps);
+  }
+  return r;
 }
 
 template <typename T>
@@ -301,6 +324,16 @@ inline void linearize_factor(int type, const std::vector<const double*>& a, Line
       auto r = res_odometry(pose_perturbed<12>(a[0], 0), pose_perturbed<12>(a[1], 6), a[2], a[3],
                             a[4][0]);
       unpack<12, 6>(r, out);
+      break;
+    }
+    case SYMHIP_FACTOR_BARRON_VECTOR5: {
+      std::array<Dual<5>, 5> x;
+      for (int i = 0; i < 5; ++i) {
+        x[i] = Dual<5>(a[0][i]);
+        x[i].d[i] = 1.0;
+      }
+      auto r = res_barron<Dual<5>, 5>(x, a[1], a[2][0], a[3][0]);
+      unpack<5, 5>(r, out);
       break;
     }
     default:

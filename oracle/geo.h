@@ -112,6 +112,10 @@ inline Dual<N> acos(const Dual<N>& a) {
 }
 template <int N>
 inline Dual<N> log(const Dual<N>& a) { return apply1(a, std::log(a.v), 1.0 / a.v); }
+template <int N>
+inline Dual<N> pow(const Dual<N>& a, double p) {
+  return apply1(a, std::pow(a.v, p), p * std::pow(a.v, p - 1.0));
+}
 // d|x| = sign(x) dx, sign(0) = 0 (matches the generated code's sign() expansion)
 template <int N>
 inline Dual<N> fabs(const Dual<N>& a) {

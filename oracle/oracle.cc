@@ -715,10 +715,17 @@ struct LM {
     b.have_cached_error = false;
   }
 
-  void Reset(const std::vector<double>& values) {  // levenberg_marquardt_solver.h:163-183
+  void Reset(const std::vector<double>& values) {  // levenberg_marquardt_solver.h:163-168
     lambda = p.initial_lambda;
     nu = p.dynamic_lambda_update_beta;
     iteration = -1;
+    ResetState(values);
+  }
+  void RelaxDampingToInitial() {  // levenberg_marquardt_solver.h:171-174
+    lambda = std::min(lambda, p.initial_lambda);
+    nu = p.dynamic_lambda_update_beta;
+  }
+  void ResetState(const std::vector<double>& values) {  // levenberg_marquardt_solver.h:178-183
     have_max_diagonal = false;
     have_last_update = false;
     // LevenbergMarquardtState::ResetValues + StateBase::Reset
@@ -1189,6 +1196,74 @@ int oracle_optimize(oracle_problem* op, const symhip_optimizer_params* params, i
     for (auto& u : ulog)
       for (double v : u) updates_out[o++] = v;
   }
+  return 0;
+  ORACLE_CATCH(-1)
+}
+
+// GncOptimizer::Optimize (opt/gnc_optimizer.h:49-118) over the Optimizer above; each stage is
+// IterateToConvergenceImpl (opt/internal/optimizer_utils.h:36-78).
+int oracle_optimize_gnc(oracle_problem* op, const symhip_optimizer_params* params, int solver,
+                        double epsilon, int32_t num_iterations, const symhip_gnc_params* g,
+                        int32_t mu_key, double* values_inout, symhip_iteration_stats* iters,
+                        int32_t max_iters, symhip_optimization_stats* out_stats) {
+  ORACLE_TRY
+  const Problem& pr = op->p;
+  if (mu_key < 0 || mu_key >= (int32_t)pr.storage_offset.size())
+    throw std::runtime_error("gnc: mu key index out of range");
+  if (pr.types[mu_key] != SYMHIP_TYPE_SCALAR) throw std::runtime_error("gnc: mu must be a Scalar");
+  const int64_t mu_off = pr.storage_offset[mu_key];
+  LM lm;
+  lm.prob = &pr;
+  lm.p = *params;
+  lm.epsilon = epsilon;
+  lm.solver.kind = solver;
+  int32_t n = num_iterations < 0 ? params->iterations : num_iterations;
+  if (n <= 0) throw std::runtime_error("num_iterations must be positive");
+  bool updating = g->mu_initial < g->mu_max && g->mu_step > 0.0;
+  std::vector<double> values(values_inout, values_inout + pr.storage_total);
+  values[mu_off] = g->mu_initial;
+  const double early_exit_min_reduction = params->early_exit_min_reduction;
+  if (updating) lm.p.early_exit_min_reduction = g->gnc_update_min_reduction;
+  lm.Reset(values);
+  std::vector<symhip_iteration_stats> stats;
+  int32_t best_index = 0;
+  int status = SYMHIP_STATUS_INVALID, reason = 0;
+  auto iterate_to_convergence = [&](int32_t count) {
+    int i;
+    for (i = 0; i < count; ++i) {
+      auto st = lm.Iterate(stats, best_index);
+      if (st) {
+        status = st->first;
+        reason = st->second;
+        break;
+      }
+    }
+    if (i == count) {
+      status = SYMHIP_STATUS_HIT_ITERATION_LIMIT;
+      reason = 0;
+    }
+    values = lm.Best().values;  // GetBestValues
+  };
+  iterate_to_convergence(n);
+  while ((int32_t)stats.size() < n) {
+    if (status != SYMHIP_STATUS_SUCCESS) break;
+    if (!updating) break;
+    values[mu_off] = values[mu_off] + g->mu_step;
+    lm.RelaxDampingToInitial();
+    if (values[mu_off] >= g->mu_max) {
+      values[mu_off] = g->mu_max;
+      lm.p.early_exit_min_reduction = early_exit_min_reduction;
+      updating = false;
+    }
+    lm.ResetState(values);  // OptimizeContinue (:121-131)
+    iterate_to_convergence(n - (int32_t)stats.size());
+  }
+  std::copy(values.begin(), values.end(), values_inout);
+  out_stats->num_iterations = (int32_t)stats.size();
+  out_stats->best_index = best_index;
+  out_stats->status = status;
+  out_stats->failure_reason = reason;
+  for (size_t k = 0; k < stats.size() && (int32_t)k < max_iters; ++k) iters[k] = stats[k];
   return 0;
   ORACLE_CATCH(-1)
 }

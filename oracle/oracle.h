@@ -85,6 +85,14 @@ int oracle_optimize_timed(oracle_problem* p, const symhip_optimizer_params* para
                           symhip_iteration_stats* iters, int32_t max_iters,
                           symhip_optimization_stats* out_stats, double* iter_seconds);
 
+/* GncOptimizer<Optimizer>::Optimize (opt/gnc_optimizer.h:49-118): stages of oracle_optimize's
+   LM (natural ordering) with the Scalar key mu_key (index into the problem's keys) raised from
+   g->mu_initial by g->mu_step to g->mu_max after each stage ending in SUCCESS. */
+int oracle_optimize_gnc(oracle_problem* p, const symhip_optimizer_params* params, int solver,
+                        double epsilon, int32_t num_iterations, const symhip_gnc_params* g,
+                        int32_t mu_key, double* values_inout, symhip_iteration_stats* iters,
+                        int32_t max_iters, symhip_optimization_stats* out_stats);
+
 /* Default parameters (optimizer.cc:8-56). */
 void oracle_default_params(symhip_optimizer_params* p);
 

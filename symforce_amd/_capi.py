@@ -19,7 +19,7 @@ PROBLEMS_LIB_PATH = BUILD_DIR / "libsymhip_problems.so"
 
 # ---- enums (include/symhip_types.h) ---------------------------------------------------------
 TYPE_SCALAR, TYPE_ROT3, TYPE_POSE3 = 1, 3, 5
-TYPE_VECTOR2, TYPE_VECTOR3, TYPE_VECTOR4, TYPE_VECTOR6 = 12, 13, 14, 16
+TYPE_VECTOR2, TYPE_VECTOR3, TYPE_VECTOR4, TYPE_VECTOR5, TYPE_VECTOR6 = 12, 13, 14, 15, 16
 TYPE_MATRIX33, TYPE_MATRIX66 = 37, 64
 
 FACTOR_PRIOR_POSE3 = 1
@@ -32,13 +32,14 @@ FACTOR_REPROJ_CAUCHY = 7
 FACTOR_MATCHING = 8
 FACTOR_ODOMETRY = 9
 FACTOR_BETWEEN_POSE3_CAUCHY = 10
+FACTOR_BARRON_VECTOR5 = 11
 
 LAMBDA_STATIC, LAMBDA_DYNAMIC = 1, 2
 STATUS_INVALID, STATUS_SUCCESS, STATUS_HIT_ITERATION_LIMIT, STATUS_FAILED = 0, 1, 2, 3
 FAILURE_INVALID, FAILURE_LAMBDA_OUT_OF_BOUNDS, FAILURE_INITIAL_ERROR_NOT_FINITE = 0, 1, 2
 
-STORAGE_DIM = {1: 1, 3: 4, 5: 7, 12: 2, 13: 3, 14: 4, 16: 6, 37: 9, 64: 36}
-TANGENT_DIM = {1: 1, 3: 3, 5: 6, 12: 2, 13: 3, 14: 4, 16: 6, 37: 9, 64: 36}
+STORAGE_DIM = {1: 1, 3: 4, 5: 7, 12: 2, 13: 3, 14: 4, 15: 5, 16: 6, 37: 9, 64: 36}
+TANGENT_DIM = {1: 1, 3: 3, 5: 6, 12: 2, 13: 3, 14: 4, 15: 5, 16: 6, 37: 9, 64: 36}
 
 I32P = C.POINTER(C.c_int32)
 I64P = C.POINTER(C.c_int64)
@@ -116,6 +117,16 @@ class IterationStats(C.Structure):
         ("new_error", C.c_double),
         ("relative_reduction", C.c_double),
         ("update_angle_change", C.c_double),
+    ]
+
+
+class GncParams(C.Structure):
+    """optimizer_gnc_params_t (lcmtypes/symforce.lcm:203-227)."""
+    _fields_ = [
+        ("gnc_update_min_reduction", C.c_double),
+        ("mu_initial", C.c_double),
+        ("mu_step", C.c_double),
+        ("mu_max", C.c_double),
     ]
 
 
@@ -410,6 +421,9 @@ _EXTRA_SIGNATURES: list = [
     ("symhip_optimizer_destroy", [_vp], C.c_int),
     ("symhip_optimizer_optimize", [_vp, F64P, C.c_int32, C.POINTER(IterationStats), C.c_int32,
                                    C.POINTER(OptimizationStats)], C.c_int),
+    ("symhip_optimizer_optimize_gnc", [_vp, F64P, C.c_int32, C.POINTER(GncParams), C.c_int32,
+                                       C.POINTER(IterationStats), C.c_int32,
+                                       C.POINTER(OptimizationStats)], C.c_int),
     ("symhip_optimizer_best_linearization", [_vp, F64P, F64P, F64P], C.c_int),
     ("symhip_optimizer_update_params", [_vp, C.POINTER(OptimizerParams)], C.c_int),
     ("symhip_optimizer_set_values", [_vp, F64P], C.c_int),
@@ -545,6 +559,20 @@ class HipOptimizer:
         st = OptimizationStats()
         check(hip_lib().symhip_optimizer_optimize(self.h, ptr(vals, C.c_double), num_iterations,
                                                   iters, maxit, C.byref(st)))
+        return dict(values=vals, status=st.status, failure_reason=st.failure_reason,
+                    best_index=st.best_index,
+                    iterations=iteration_records(iters, st.num_iterations))
+
+    def optimize_gnc(self, gnc, mu_key, values=None, num_iterations=-1):
+        """GncOptimizer::Optimize (opt/gnc_optimizer.h:49-118); mu_key indexes the problem's
+        keys."""
+        vals = (self.prob.values if values is None else np.asarray(values, np.float64)).copy()
+        maxit = 4096
+        iters = (IterationStats * maxit)()
+        st = OptimizationStats()
+        check(hip_lib().symhip_optimizer_optimize_gnc(self.h, ptr(vals, C.c_double),
+                                                      num_iterations, C.byref(gnc), mu_key,
+                                                      iters, maxit, C.byref(st)))
         return dict(values=vals, status=st.status, failure_reason=st.failure_reason,
                     best_index=st.best_index,
                     iterations=iteration_records(iters, st.num_iterations))

@@ -201,6 +201,19 @@ int symhip_optimizer_optimize(symhip_optimizer_t o, double* values, int32_t num_
   });
 }
 
+int symhip_optimizer_optimize_gnc(symhip_optimizer_t o, double* values, int32_t num_iterations,
+                                  const symhip_gnc_params* gnc, int32_t mu_key_index,
+                                  symhip_iteration_stats* iters, int32_t max_iters,
+                                  symhip_optimization_stats* stats) {
+  return guarded([&] {
+    SYMHIP_CHECK(gnc != nullptr && values != nullptr, "null argument");
+    symhip_optimization_stats st{};
+    o->opt->optimize_gnc(values, num_iterations, mu_key_index, *gnc, o->iters, st);
+    copy_stats(o->iters, iters, max_iters);
+    if (stats) *stats = st;
+  });
+}
+
 int symhip_optimizer_best_linearization(symhip_optimizer_t o, double* residual,
                                         double* hessian_values, double* rhs) {
   return guarded([&] { o->opt->best_linearization(residual, hessian_values, rhs); });

@@ -90,6 +90,7 @@ inline int storage_dim(int type) {
     case SYMHIP_TYPE_VECTOR2: return 2;
     case SYMHIP_TYPE_VECTOR3: return 3;
     case SYMHIP_TYPE_VECTOR4: return 4;
+    case SYMHIP_TYPE_VECTOR5: return 5;
     case SYMHIP_TYPE_VECTOR6: return 6;
     case SYMHIP_TYPE_MATRIX33: return 9;
     case SYMHIP_TYPE_MATRIX66: return 36;
@@ -114,7 +115,8 @@ struct FactorSpec {
 inline FactorSpec factor_spec(int type) {
   const int P = SYMHIP_TYPE_POSE3, R = SYMHIP_TYPE_ROT3, S = SYMHIP_TYPE_SCALAR,
             V2 = SYMHIP_TYPE_VECTOR2, V3 = SYMHIP_TYPE_VECTOR3, V4 = SYMHIP_TYPE_VECTOR4,
-            V6 = SYMHIP_TYPE_VECTOR6, M3 = SYMHIP_TYPE_MATRIX33, M6 = SYMHIP_TYPE_MATRIX66;
+            V5 = SYMHIP_TYPE_VECTOR5, V6 = SYMHIP_TYPE_VECTOR6, M3 = SYMHIP_TYPE_MATRIX33,
+            M6 = SYMHIP_TYPE_MATRIX66;
   switch (type) {
     case SYMHIP_FACTOR_PRIOR_POSE3: return {4, {P, P, M6, S}, 0b1, 6, 6};
     case SYMHIP_FACTOR_BETWEEN_POSE3: return {5, {P, P, P, M6, S}, 0b11, 6, 12};
@@ -126,9 +128,10 @@ inline FactorSpec factor_spec(int type) {
     case SYMHIP_FACTOR_MATCHING: return {4, {P, V3, V3, S}, 0b1, 3, 6};
     case SYMHIP_FACTOR_ODOMETRY: return {5, {P, P, P, V6, S}, 0b11, 6, 12};
     case SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY: return {6, {P, P, P, M6, S, S}, 0b11, 6, 12};
+    case SYMHIP_FACTOR_BARRON_VECTOR5: return {4, {V5, V5, S, S}, 0b1, 5, 5};
   }
   throw Error("symhip: unknown factor type " + std::to_string(type));
 }
-constexpr int kNumFactorTypes = 11;  // valid codes are 1..10
+constexpr int kNumFactorTypes = 12;  // valid codes are 1..11
 
 }  // namespace symhip

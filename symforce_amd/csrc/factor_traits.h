@@ -31,6 +31,7 @@ SYMHIP_FT(SYMHIP_FACTOR_MATCHING, 3, 6, 4, 1, 6, 0, 0, factor_matching<S>(a, r, 
 SYMHIP_FT(SYMHIP_FACTOR_ODOMETRY, 6, 12, 5, 2, 6, 6, 0, factor_odometry<S>(a, r, J))
 SYMHIP_FT(SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY, 6, 12, 6, 2, 6, 6, 0,
           factor_between_pose3<S>(a, r, J, 1))
+SYMHIP_FT(SYMHIP_FACTOR_BARRON_VECTOR5, 5, 5, 4, 1, 5, 0, 0, factor_barron5<S>(a, r, J))
 
 #undef SYMHIP_FT
 
@@ -50,6 +51,7 @@ inline void dispatch_factor_type(int code, F&& f) {
     case SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY:
       f.template operator()<SYMHIP_FACTOR_BETWEEN_POSE3_CAUCHY>();
       break;
+    case SYMHIP_FACTOR_BARRON_VECTOR5: f.template operator()<SYMHIP_FACTOR_BARRON_VECTOR5>(); break;
     default: throw Error("symhip: unknown factor type");
   }
 }

@@ -181,6 +181,7 @@ SYMHIP_HD void retract_value(int type, double* v, const double* d, double eps) {
                    : type == SYMHIP_TYPE_VECTOR2  ? 2
                    : type == SYMHIP_TYPE_VECTOR3  ? 3
                    : type == SYMHIP_TYPE_VECTOR4  ? 4
+                   : type == SYMHIP_TYPE_VECTOR5  ? 5
                    : type == SYMHIP_TYPE_VECTOR6  ? 6
                    : type == SYMHIP_TYPE_MATRIX33 ? 9
                                                   : 36;
@@ -492,6 +493,32 @@ SYMHIP_HD void factor_odometry(const double* const* a, S* r, S (*J)[12]) {
     const S inv = S(1) / S(sig[i]);
     r[i] = e[i] * inv;
     for (int c = 0; c < 12; ++c) J[i][c] = de[i][c] * inv;
+  }
+}
+
+// Graduated non-convexity factor: per element e = x_i - y_i, the Barron loss
+//   rho(e) = (b / d) ((e^2 / b + 1)^(d / 2) - 1),  b = |alpha - 2| + eps,
+//   d = alpha + eps (alpha >= 0) or alpha - eps,  alpha = 2 - 1 / (1 - mu + eps)
+// (delta 1, unit information; symforce/opt/noise_models.py:382-436), whitened as
+// r_i = sqrt(2 rho(e) + eps) - sqrt(eps). Its derivative in x_i:
+// drho/de = e (e^2 / b + 1)^(d / 2 - 1), so dr_i/dx_i = drho/de / sqrt(2 rho + eps). (At
+// alpha -> 0, mu -> 1/2, the form cancels: t^(d/2) - 1 is a few ulps.)
+template <typename S>
+SYMHIP_HD void factor_barron5(const double* const* a, S* r, S (*J)[5]) {
+  const double mu = a[2][0], eps = a[3][0];
+  const double alpha = 2.0 - 1.0 / (1.0 - mu + eps);
+  const double b = fabs(alpha - 2.0) + eps;
+  const double d = alpha >= 0.0 ? alpha + eps : alpha - eps;
+  const double se = sqrt(eps);
+  for (int i = 0; i < 5; ++i) {
+    const double e = a[0][i] - a[1][i];
+    const double t = e * e / b + 1.0;
+    const double pd = pow(t, 0.5 * d);  // t^(d/2)
+    const double rho = (b / d) * (pd - 1.0);
+    const double w = sqrt(2.0 * rho + eps);
+    r[i] = S(w - se);
+    for (int c = 0; c < 5; ++c) J[i][c] = S(0);
+    J[i][i] = S(e * (pd / t) / w);
   }
 }
 

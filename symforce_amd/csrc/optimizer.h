@@ -1,6 +1,7 @@
 // Device-resident Levenberg-Marquardt optimizer (sym::Optimizer + LevenbergMarquardtSolver).
 #pragma once
 
+#include <chrono>
 #include <memory>
 #include <vector>
 
@@ -116,6 +117,12 @@ class DeviceOptimizer {
   // Same, with Values already on the device (benchmark path: no host transfers in the loop).
   void optimize_device(int32_t num_iterations, std::vector<symhip_iteration_stats>& iters,
                        symhip_optimization_stats& stats);
+  // GncOptimizer::Optimize (opt/gnc_optimizer.h:49-118): stages of Levenberg-Marquardt to
+  // convergence with the scalar key mu_key (a constant of the factors) stepped from mu_initial to
+  // mu_max between them; the iteration records and the budget run across the stages.
+  void optimize_gnc(double* values_inout, int32_t num_iterations, int32_t mu_key,
+                    const symhip_gnc_params& g, std::vector<symhip_iteration_stats>& iters,
+                    symhip_optimization_stats& stats);
   void upload_values(const double* values);
   void download_best_values(double* values);
   void best_linearization(double* residual, double* hval, double* rhs);
@@ -137,6 +144,11 @@ class DeviceOptimizer {
  private:
   bool iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index, int& status,
                int& reason);
+  // LevenbergMarquardtSolver::Reset (levenberg_marquardt_solver.h:163-168) with the start Values.
+  void reset_run();
+  // IterateToConvergenceImpl (opt/internal/optimizer_utils.h:36-78) minus the Values copy.
+  void iterate_to_convergence(int32_t n, std::vector<symhip_iteration_stats>& iters,
+                              int32_t& best_index, int& status, int& reason);
   void record_debug(int block, bool with_update);
   std::vector<DebugRecord> debug_;
   void relinearize(int b);
@@ -158,6 +170,7 @@ class DeviceOptimizer {
   bool have_max_diagonal_ = false, have_last_update_ = false;
   int iteration_ = -1;
   double last_iter_ms_ = 0;
+  std::chrono::steady_clock::time_point run_start_;
   std::vector<double> iter_times_;
   DBuf<double> start_values_;
   bool have_start_ = false;

@@ -38,6 +38,14 @@ __global__ void lm_reset_kernel(LmDevState* st, double lambda, double nu) {
   st->set_best = 0;
 }
 
+// RelaxDampingToInitial (levenberg_marquardt_solver.h:171-174).
+__global__ void lm_relax_kernel(LmDevState* st, double initial_lambda, double beta) {
+  st->lambda = fmin(st->lambda, initial_lambda);
+  st->nu = beta;
+}
+
+__global__ void set_value_kernel(double* p, double v) { *p = v; }
+
 // Record the iteration -1 statistics (levenberg_marquardt_solver.tcc:158-186).
 __global__ void lm_first_record_kernel(LmDevState* st, int init) {
   st->rec = symhip_iteration_stats{};
@@ -322,12 +330,7 @@ void DeviceOptimizer::optimize(double* values_inout, int32_t num_iterations, boo
   (void)populate_best;
 }
 
-void DeviceOptimizer::optimize_device(int32_t num_iterations,
-                                      std::vector<symhip_iteration_stats>& iters,
-                                      symhip_optimization_stats& stats) {
-  const int32_t n = num_iterations < 0 ? p_.iterations : num_iterations;
-  SYMHIP_CHECK(n > 0, "num_iterations must be positive");
-  // Reset (levenberg_marquardt_solver.h:163-183): the start Values go into the New block.
+void DeviceOptimizer::reset_run() {
   SYMHIP_CHECK(have_start_, "optimize called before the Values were set");
   HIP_OK(hipMemcpyAsync(values_[1].get(), start_values_.get(), sizeof(double) * host_.storage_total,
                         hipMemcpyDeviceToDevice, stream_));
@@ -345,19 +348,21 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
                      p_.dynamic_lambda_update_beta);
   backend_->begin_run();
   debug_.clear();
-  iters.clear();
   iter_times_.clear();
-  int32_t best_index = 0;
-  int status = 0, reason = 0;
+  run_start_ = std::chrono::steady_clock::now();
+}
+
+void DeviceOptimizer::iterate_to_convergence(int32_t n, std::vector<symhip_iteration_stats>& iters,
+                                             int32_t& best_index, int& status, int& reason) {
+  SYMHIP_CHECK(n > 0, "num_iterations must be positive");
   int i;
-  const auto t_start = std::chrono::steady_clock::now();
   for (i = 0; i < n; ++i) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t before = iters.size();
     const bool done = iterate(iters, best_index, status, reason);
     const auto t1 = std::chrono::steady_clock::now();
     last_iter_ms_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    const double since = std::chrono::duration<double, std::milli>(t1 - t_start).count();
+    const double since = std::chrono::duration<double, std::milli>(t1 - run_start_).count();
     for (size_t k = before; k < iters.size(); ++k) iter_times_.push_back(since);
     if (done) break;
   }
@@ -365,6 +370,84 @@ void DeviceOptimizer::optimize_device(int32_t num_iterations,
     status = SYMHIP_STATUS_HIT_ITERATION_LIMIT;
     reason = 0;
   }
+}
+
+void DeviceOptimizer::optimize_device(int32_t num_iterations,
+                                      std::vector<symhip_iteration_stats>& iters,
+                                      symhip_optimization_stats& stats) {
+  const int32_t n = num_iterations < 0 ? p_.iterations : num_iterations;
+  SYMHIP_CHECK(n > 0, "num_iterations must be positive");
+  // Reset (levenberg_marquardt_solver.h:163-183): the start Values go into the New block.
+  reset_run();
+  iters.clear();
+  int32_t best_index = 0;
+  int status = 0, reason = 0;
+  iterate_to_convergence(n, iters, best_index, status, reason);
+  stats.num_iterations = (int32_t)iters.size();
+  stats.best_index = best_index;
+  stats.status = status;
+  stats.failure_reason = reason;
+}
+
+void DeviceOptimizer::optimize_gnc(double* values_inout, int32_t num_iterations, int32_t mu_key,
+                                   const symhip_gnc_params& g,
+                                   std::vector<symhip_iteration_stats>& iters,
+                                   symhip_optimization_stats& stats) {
+  SYMHIP_CHECK(mu_key >= 0 && mu_key < host_.n_keys, "gnc: mu key index out of range");
+  SYMHIP_CHECK(host_.types[mu_key] == SYMHIP_TYPE_SCALAR, "gnc: the mu key must be a Scalar");
+  SYMHIP_CHECK(host_.tan_off[mu_key] < 0, "gnc: the mu key must not be optimized");
+  const int32_t n = num_iterations < 0 ? p_.iterations : num_iterations;
+  SYMHIP_CHECK(n > 0, "num_iterations must be positive");
+  const int64_t mu_off = host_.storage_offset[mu_key];
+  bool updating = g.mu_initial < g.mu_max && g.mu_step > 0.0;
+  values_inout[mu_off] = g.mu_initial;
+  const symhip_optimizer_params user = p_;
+  if (updating) p_.early_exit_min_reduction = g.gnc_update_min_reduction;
+  upload_values(values_inout);
+  init_ = 0;
+  new_ = 1;
+  best_ = 0;
+  free_ = 2;
+  reset_run();
+  iters.clear();
+  int32_t best_index = 0;
+  int status = 0, reason = 0;
+  try {
+    iterate_to_convergence(n, iters, best_index, status, reason);
+    double mu = g.mu_initial;  // the stage's best Values hold it
+    while ((int32_t)iters.size() < n) {
+      if (status != SYMHIP_STATUS_SUCCESS || !updating) break;
+      mu += g.mu_step;
+      // RelaxDampingToInitial (levenberg_marquardt_solver.h:171-174)
+      hipLaunchKernelGGL(lm_relax_kernel, dim3(1), dim3(1), 0, stream_, dstate_,
+                         user.initial_lambda, user.dynamic_lambda_update_beta);
+      if (mu >= g.mu_max) {
+        mu = g.mu_max;
+        p_.early_exit_min_reduction = user.early_exit_min_reduction;
+        updating = false;
+      }
+      // OptimizeContinue: ResetState(best Values with the new mu) (:121-131,
+      // levenberg_marquardt_solver.h:178-183): the Values go into the New block, every
+      // linearization is dropped, lambda and the iteration count stay.
+      if (best_ != new_)
+        HIP_OK(hipMemcpyAsync(values_[new_].get(), values_[best_].get(),
+                              sizeof(double) * host_.storage_total, hipMemcpyDeviceToDevice,
+                              stream_));
+      hipLaunchKernelGGL(set_value_kernel, dim3(1), dim3(1), 0, stream_,
+                         values_[new_].get() + mu_off, mu);
+      have_max_diagonal_ = false;
+      have_last_update_ = false;
+      for (int b = 0; b < 3; ++b) linearized_[b] = false;
+      valid_[new_] = true;
+      valid_[init_] = valid_[free_] = false;
+      iterate_to_convergence(n - (int32_t)iters.size(), iters, best_index, status, reason);
+    }
+  } catch (...) {
+    p_ = user;
+    throw;
+  }
+  p_ = user;
+  download_best_values(values_inout);
   stats.num_iterations = (int32_t)iters.size();
   stats.best_index = best_index;
   stats.status = status;

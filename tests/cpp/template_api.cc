@@ -11,6 +11,8 @@
 //   covariance                Linearize, ComputeFullCovariance, ComputeAllCovariances and the
 //                             Schur-complement ComputeCovariances on a pose graph
 //   optimizerf                sym::Optimizerf on the reference pose-smoothing problem
+//   gnc                       sym::GncOptimizer<sym::Optimizerd> on the reference GNC problem
+//                             (test/symforce_gnc_test.cc:13-80), the same random draws
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -282,6 +284,60 @@ int optimizerf() {
   return 0;
 }
 
+// "Test GNC" (test/symforce_gnc_test.cc:22-80): 20 Barron residuals on a Vector5 x, 3 of them
+// outliers, mu raised from 0 by 0.33 to 0.99; the samples are drawn as the reference draws them
+// (std::mt19937(42), a fresh std::normal_distribution per Random<Vector5d> call).
+int gnc() {
+  constexpr double kEpsilon = 1e-12;
+  const int n_residuals = 20, n_outliers = 3;
+  std::mt19937 gen(42);
+  auto random5 = [&]() {
+    std::normal_distribution<double> dist{};
+    sym::Vector5d v;
+    for (int i = 0; i < 5; ++i) v.data[i] = dist(gen);
+    return v;
+  };
+  sym::Valuesd initial_values;
+  sym::Vector5d ones;
+  ones.data.fill(1.0);
+  initial_values.Set('x', ones);
+  initial_values.Set('e', sym::kDefaultEpsilond);
+  std::vector<double> ys;
+  for (int i = 0; i < n_residuals; ++i) {
+    sym::Vector5d y = random5();
+    for (int k = 0; k < 5; ++k) y.data[k] = (i < n_outliers ? 10.0 : 0.0) + 0.1 * y.data[k];
+    initial_values.Set(sym::Key('y', i), y);
+    ys.insert(ys.end(), y.data.begin(), y.data.end());
+  }
+  std::vector<sym::Factord> factors;
+  for (int i = 0; i < n_residuals; ++i)
+    factors.push_back(sym::Factord::Hessian(sym::BarronFactor<double>,
+                                            {'x', sym::Key('y', i), 'u', 'e'}, {'x'}));
+  auto params = sym::DefaultOptimizerParams();
+  sym::optimizer_gnc_params_t gnc_params{};
+  gnc_params.mu_initial = 0;
+  gnc_params.mu_max = 0.99;
+  gnc_params.mu_step = 0.33;
+  gnc_params.gnc_update_min_reduction = 1e-3;
+  sym::GncOptimizer<sym::Optimizerd> gnc_optimizer(params, gnc_params, 'u', factors,
+                                                   "sym::Optimize", std::vector<sym::Key>{},
+                                                   kEpsilon);
+  sym::Valuesd gnc_values = initial_values;
+  const auto gnc_stats = gnc_optimizer.Optimize(gnc_values);
+  sym::Valuesd regular_values = initial_values;
+  regular_values.Set('u', 0.0);
+  sym::Optimize(params, factors, regular_values, kEpsilon);
+  const auto xg = gnc_values.At<sym::Vector5d>('x');
+  const auto xr = regular_values.At<sym::Vector5d>('x');
+  std::printf("{\"iterations\": %s, \"status\": %d, \"best_index\": %d, \"x_gnc\": %s, "
+              "\"x_regular\": %s, \"mu_final\": %s, \"ys\": %s, \"values\": %s}\n",
+              iterations_json(gnc_stats).c_str(), gnc_stats.status, gnc_stats.best_index,
+              arr(xg.data, 5).c_str(), arr(xr.data, 5).c_str(),
+              num(gnc_values.AtScalar('u')).c_str(), arr(ys, ys.size()).c_str(),
+              arr(gnc_values.Data(), gnc_values.Data().size()).c_str());
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -294,6 +350,7 @@ int main(int argc, char** argv) {
     if (mode == "debug_stats") return debug_stats();
     if (mode == "covariance") return covariance();
     if (mode == "optimizerf") return optimizerf();
+    if (mode == "gnc") return gnc();
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -61,6 +61,10 @@ def lib(fast_math=False):
                                             capi.I32P, C.c_int32, C.c_double, C.c_int32,
                                             capi.F64P, C.POINTER(capi.IterationStats), C.c_int32,
                                             C.POINTER(capi.OptimizationStats), capi.F64P]
+        L.oracle_optimize_gnc.argtypes = [vp, C.POINTER(capi.OptimizerParams), C.c_int,
+                                          C.c_double, C.c_int32, C.POINTER(capi.GncParams),
+                                          C.c_int32, capi.F64P, C.POINTER(capi.IterationStats),
+                                          C.c_int32, C.POINTER(capi.OptimizationStats)]
         L.oracle_linearize_factor.argtypes = [C.c_int32, C.POINTER(capi.F64P), capi.F64P,
                                               capi.F64P, capi.F64P, capi.F64P]
         _libs[path] = L
@@ -147,6 +151,24 @@ class OracleProblem:
         if record_updates:
             out["updates"] = upd[: (st.num_iterations - 1) * self.N].reshape(-1, self.N)
         return out
+
+
+    def optimize_gnc(self, gnc, mu_key, params=None, solver=CHOLESKY, epsilon=None,
+                     num_iterations=-1, values=None):
+        params = params or capi.default_params()
+        eps = self.prob.info.get("epsilon", 2.220446049250313e-15) if epsilon is None else epsilon
+        vals = (self.prob.values if values is None else np.asarray(values, np.float64)).copy()
+        maxit = 4096
+        iters = (capi.IterationStats * maxit)()
+        st = capi.OptimizationStats()
+        rc = self.L.oracle_optimize_gnc(self.h, C.byref(params), solver, eps, num_iterations,
+                                       C.byref(gnc), mu_key, _p(vals, C.c_double), iters, maxit,
+                                       C.byref(st))
+        if rc != 0:
+            raise RuntimeError(self.L.oracle_last_error().decode())
+        return dict(values=vals, status=st.status, failure_reason=st.failure_reason,
+                    best_index=st.best_index,
+                    iterations=capi.iteration_records(iters, st.num_iterations))
 
 
 def time_iterations(op: "OracleProblem", params, num_iterations, solver=CHOLESKY, c_dim=0,

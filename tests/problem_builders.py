@@ -63,3 +63,24 @@ def robot3d_problem() -> tuple[capi.Problem, dict]:
     for i in range(n - 1):
         b.factor(capi.FACTOR_ODOMETRY, [poses[i], poses[i + 1], odo[i], sig, eps])
     return b.build(), d
+
+
+def gnc_problem(ys, x0=None) -> tuple[capi.Problem, int]:
+    """The reference's GNC problem (test/symforce_gnc_test.cc:26-48): Barron factors on a
+    Vector5 'x' against constant samples ('y', i), with the convexity parameter 'u' and the
+    factor epsilon 'e'; keys in the order the reference's Values hold them ('u' is appended by
+    GncOptimizer::Optimize). Returns the problem and the key index of 'u'."""
+    b = ProblemBuilder()
+    x = b.key("x", type_=capi.TYPE_VECTOR5, value=np.ones(5) if x0 is None else x0)
+    e = b.key("e", value=10 * np.finfo(np.float64).eps)
+    yk = [b.key("y", i, INVALID, capi.TYPE_VECTOR5, y) for i, y in enumerate(ys)]
+    u = b.key("u", value=0.0)
+    for k in yk:
+        b.factor(capi.FACTOR_BARRON_VECTOR5, [x, k, u, e])
+    b.info["epsilon"] = 1e-12  # the optimizer epsilon the reference test passes
+    return b.build(), u
+
+
+def gnc_reference_samples() -> np.ndarray:
+    """The 20 samples of the reference's GNC test (tests/golden/gnc_problem.json)."""
+    return np.array(json.loads((GOLDEN / "gnc_problem.json").read_text())["ys"])
