@@ -1212,6 +1212,16 @@ class Optimizer {
                                       (int32_t)recs.size(), &st));
     for (size_t i = 0; i < values.Data().size(); ++i) values.Data()[i] = static_cast<Scalar>(v[i]);
     const index_t all = values.CreateIndex();
+    std::vector<int32_t> jcp, jri;
+    if (p.include_jacobians) {
+      const int64_t jnnz = symhip_optimizer_jacobian_sparsity(opt_, nullptr, nullptr);
+      if (jnnz < 0) check((int)jnnz);
+      jcp.resize(N_ + 1);
+      jri.resize(jnnz);
+      check((int)std::min<int64_t>(0, symhip_optimizer_jacobian_sparsity(opt_, jcp.data(),
+                                                                          jri.data())));
+      if (p.debug_stats) stats.jacobian_sparsity = {jri, jcp, {{M_, N_}}};
+    }
     for (int32_t i = 0; i < st.num_iterations; ++i) {
       optimization_iteration_t it = FromRecord(recs[i]);
       if (p.debug_stats) {
@@ -1221,6 +1231,12 @@ class Optimizer {
         it.values.index = all;
         it.values.data = vals;
         it.residual = ToVector(res);
+        if (p.include_jacobians) {
+          // optimization_iteration_t.jacobian_values: the Jacobian's nonzeros, CSC order
+          std::vector<double> jv(jri.size());
+          check(symhip_optimizer_debug_jacobian(opt_, i, jv.data()));
+          it.jacobian_values = ToVector(jv);
+        }
       }
       stats.iterations.push_back(std::move(it));
     }
@@ -1245,6 +1261,11 @@ class Optimizer {
       std::vector<double> r(M_), h(nnz_), b(N_);
       check(symhip_optimizer_best_linearization(opt_, r.data(), h.data(), b.data()));
       FillLinearization(lin, r, h, b);
+      if (p.include_jacobians) {
+        std::vector<double> jv(jri.size());
+        check(symhip_optimizer_best_jacobian(opt_, jv.data()));
+        lin.jacobian = MatrixType(M_, N_, jcp, jri, std::vector<Scalar>(jv.begin(), jv.end()));
+      }
       stats.best_linearization = std::move(lin);
     }
   }

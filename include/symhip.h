@@ -283,6 +283,28 @@ int symhip_optimizer_eval_preconditioner(symhip_optimizer_t opt, int32_t idx, do
    may be NULL. Fails when the optimizer was created without debug_stats. */
 int symhip_optimizer_debug_record(symhip_optimizer_t opt, int32_t index, double* update,
                                   double* values, double* residual);
+/* The warnings of the last optimize call, each followed by the record separator 0x1e (a warning
+   may span several lines), in the wording of the reference's
+   spdlog::warn calls: zero on the damped Hessian diagonal (CheckHessianDiagonal,
+   levenberg_marquardt_solver.tcc:56-90), non-finite initial or new error (:180-183, :257-262)
+   and, with optimizer_params_t.debug_checks, the non-finite update (:225-227) and every factor
+   whose linearization is not finite, with its keys, values and residual
+   (opt/internal/linearizer_utils.h:307-367). Each is also written to stderr as it happens.
+   Copies at most cap - 1 bytes and a NUL into buf (buf may be NULL when cap is 0); returns the
+   full length in bytes, or a negative code. */
+int64_t symhip_optimizer_warnings(symhip_optimizer_t opt, char* buf, int64_t cap);
+/* include_jacobians (optimizer_params_t.include_jacobians): OptimizationStats.jacobian_sparsity
+   (levenberg_marquardt_solver.tcc:172-175), the M x N Jacobian's CSC pattern (colptr N + 1,
+   rowind nnz; rows sorted), and its values: of the best linearization of the last optimize
+   (Linearization::jacobian of GetBestLinearization), or of iteration record `index` under
+   debug_stats (optimization_iteration_t.jacobian_values, :115-122, :165-170). The sparsity call
+   returns nnz (pass NULL outputs to query it) or a negative code; all fail when the optimizer
+   was created without include_jacobians. check_derivatives requires include_jacobians, as in
+   the reference (optimizer.tcc:39). */
+int64_t symhip_optimizer_jacobian_sparsity(symhip_optimizer_t opt, int32_t* colptr,
+                                           int32_t* rowind);
+int symhip_optimizer_best_jacobian(symhip_optimizer_t opt, double* values);
+int symhip_optimizer_debug_jacobian(symhip_optimizer_t opt, int32_t index, double* values);
 /* OptimizationStats.linear_solver_ordering and .cholesky_factor_sparsity
    (levenberg_marquardt_solver.tcc:205-212): the sparse Cholesky path's Permutation().indices()
    (N entries, old -> new) and the strictly lower CSC pattern of its factor L (colptr N + 1,

@@ -425,6 +425,10 @@ _EXTRA_SIGNATURES: list = [
                                        C.POINTER(IterationStats), C.c_int32,
                                        C.POINTER(OptimizationStats)], C.c_int),
     ("symhip_optimizer_best_linearization", [_vp, F64P, F64P, F64P], C.c_int),
+    ("symhip_optimizer_warnings", [_vp, C.c_char_p, C.c_int64], C.c_int64),
+    ("symhip_optimizer_jacobian_sparsity", [_vp, I32P, I32P], C.c_int64),
+    ("symhip_optimizer_best_jacobian", [_vp, F64P], C.c_int),
+    ("symhip_optimizer_debug_jacobian", [_vp, C.c_int32, F64P], C.c_int),
     ("symhip_optimizer_update_params", [_vp, C.POINTER(OptimizerParams)], C.c_int),
     ("symhip_optimizer_set_values", [_vp, F64P], C.c_int),
     ("symhip_optimizer_optimize_resident", [_vp, C.c_int32, C.POINTER(IterationStats), C.c_int32,
@@ -576,6 +580,35 @@ class HipOptimizer:
         return dict(values=vals, status=st.status, failure_reason=st.failure_reason,
                     best_index=st.best_index,
                     iterations=iteration_records(iters, st.num_iterations))
+
+    def jacobian_sparsity(self, N):
+        """include_jacobians: the Jacobian's CSC pattern (colptr over the N columns, rowind)."""
+        L = hip_lib()
+        nnz = L.symhip_optimizer_jacobian_sparsity(self.h, None, None)
+        check(nnz if nnz < 0 else 0)
+        cp = np.zeros(N + 1, np.int32)
+        ri = np.zeros(nnz, np.int32)
+        check(min(0, L.symhip_optimizer_jacobian_sparsity(self.h, ptr(cp, C.c_int32),
+                                                          ptr(ri, C.c_int32))))
+        return cp, ri
+
+    def best_jacobian(self, nnz):
+        v = np.zeros(nnz)
+        check(hip_lib().symhip_optimizer_best_jacobian(self.h, ptr(v, C.c_double)))
+        return v
+
+    def debug_jacobian(self, index, nnz):
+        v = np.zeros(nnz)
+        check(hip_lib().symhip_optimizer_debug_jacobian(self.h, index, ptr(v, C.c_double)))
+        return v
+
+    def warnings(self):
+        """The last optimize call's warnings (symhip_optimizer_warnings), one string each."""
+        n = hip_lib().symhip_optimizer_warnings(self.h, None, 0)
+        check(n if n < 0 else 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        check(min(0, hip_lib().symhip_optimizer_warnings(self.h, buf, n + 1)))
+        return [w for w in buf.value.decode().split("\x1e") if w]
 
     def set_values(self, values):
         v = np.ascontiguousarray(values, np.float64)

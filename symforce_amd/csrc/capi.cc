@@ -276,6 +276,56 @@ int symhip_optimizer_debug_record(symhip_optimizer_t o, int32_t index, double* u
   });
 }
 
+int64_t symhip_optimizer_jacobian_sparsity(symhip_optimizer_t o, int32_t* colptr,
+                                           int32_t* rowind) {
+  int64_t nnz = -1;
+  const int rc = guarded([&] {
+    SYMHIP_CHECK(o, "invalid arguments");
+    SYMHIP_CHECK(o->opt->params().include_jacobians,
+                 "no Jacobian: create the optimizer with include_jacobians");
+    const symhip::JacobianStructure& j = o->opt->jacobian_structure();
+    nnz = (int64_t)j.rowind.size();
+    if (colptr) std::copy(j.colptr.begin(), j.colptr.end(), colptr);
+    if (rowind) std::copy(j.rowind.begin(), j.rowind.end(), rowind);
+  });
+  return rc < 0 ? rc : nnz;
+}
+
+int symhip_optimizer_best_jacobian(symhip_optimizer_t o, double* values) {
+  return guarded([&] {
+    SYMHIP_CHECK(o && values, "invalid arguments");
+    o->opt->best_jacobian(values);
+  });
+}
+
+int symhip_optimizer_debug_jacobian(symhip_optimizer_t o, int32_t index, double* values) {
+  return guarded([&] {
+    SYMHIP_CHECK(o && values, "invalid arguments");
+    const auto& recs = o->opt->debug_records();
+    SYMHIP_CHECK(!recs.empty(), "no debug records: create the optimizer with debug_stats set");
+    SYMHIP_CHECK(index >= 0 && index < (int32_t)recs.size(), "debug record index out of range");
+    SYMHIP_CHECK(!recs[index].jacobian.empty(),
+                 "no Jacobian values: create the optimizer with include_jacobians set");
+    std::copy(recs[index].jacobian.begin(), recs[index].jacobian.end(), values);
+  });
+}
+
+int64_t symhip_optimizer_warnings(symhip_optimizer_t o, char* buf, int64_t cap) {
+  int64_t len = -1;
+  const int rc = guarded([&] {
+    SYMHIP_CHECK(o && cap >= 0 && (buf || cap == 0), "invalid arguments");
+    std::string all;
+    for (const auto& w : o->opt->warnings()) all += w + "\x1e";
+    len = (int64_t)all.size();
+    if (cap > 0) {
+      const int64_t n = std::min<int64_t>(cap - 1, len);
+      std::copy(all.begin(), all.begin() + n, buf);
+      buf[n] = 0;
+    }
+  });
+  return rc < 0 ? rc : len;
+}
+
 int64_t symhip_optimizer_factor_sparsity(symhip_optimizer_t o, int32_t* ordering,
                                          int32_t* colptr, int32_t* rowind) {
   int64_t nnz = -1;

@@ -3,6 +3,7 @@
 
 #include <chrono>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "./chol_numeric.h"
@@ -44,6 +45,8 @@ class SystemBackend {
   virtual const double* rhs(int b) const = 0;
   virtual const double* residual(int b) const = 0;
   virtual const double* damping() const = 0;
+  // The Hessian diagonal before the last damp() (N, reference state order).
+  virtual const double* undamped_diagonal() const = 0;
   // Hessian in reference CSC order (for best-linearization and parity output); may be null.
   virtual void hessian_csc(int b, double* out_host, hipStream_t s) = 0;
   virtual int32_t N() const = 0;
@@ -126,8 +129,13 @@ class DeviceOptimizer {
   void upload_values(const double* values);
   void download_best_values(double* values);
   void best_linearization(double* residual, double* hval, double* rhs);
-  void update_params(const symhip_optimizer_params& p) { p_ = p; }
+  void update_params(const symhip_optimizer_params& p) {
+    SYMHIP_CHECK(!p.check_derivatives || p.include_jacobians,
+                 "check_derivatives requires include_jacobians");
+    p_ = p;
+  }
   const HostProblem& host() const { return host_; }
+  const symhip_optimizer_params& params() const { return p_; }
   SystemBackend& backend() { return *backend_; }
   hipStream_t stream() const { return stream_; }
   double last_iteration_ms() const { return last_iter_ms_; }
@@ -137,9 +145,17 @@ class DeviceOptimizer {
   // iteration -1 record and has no update): the update, the New block's Values storage and its
   // residual (PopulateIterationStats, levenberg_marquardt_solver.tcc:115-122).
   struct DebugRecord {
-    std::vector<double> update, values, residual;
+    std::vector<double> update, values, residual, jacobian;
   };
+  // include_jacobians: the Jacobian's sparsity (stats.jacobian_sparsity, lm.tcc:172-175) and the
+  // values of the best block's Jacobian (Linearization::jacobian of GetBestLinearization).
+  const JacobianStructure& jacobian_structure();
+  void best_jacobian(double* values);
   const std::vector<DebugRecord>& debug_records() const { return debug_; }
+  // Warnings of the last optimize run, in the reference's wording (spdlog::warn in
+  // levenberg_marquardt_solver.tcc:56-90, 180-183, 225-227, 257-262 and, under debug_checks,
+  // opt/internal/linearizer_utils.h:307-367); each is also written to stderr.
+  const std::vector<std::string>& warnings() const { return warnings_; }
 
  private:
   bool iterate(std::vector<symhip_iteration_stats>& iters, int32_t& best_index, int& status,
@@ -150,6 +166,20 @@ class DeviceOptimizer {
   void iterate_to_convergence(int32_t n, std::vector<symhip_iteration_stats>& iters,
                               int32_t& best_index, int& status, int& reason);
   void record_debug(int block, bool with_update);
+  void warn(const std::string& msg);
+  void check_factors(int block);
+  void check_derivatives(int block);
+  void upload_check_tables();
+  void check_iteration();
+  std::vector<std::string> warnings_;
+  DBuf<int32_t> check_type_, check_ptr_, check_bad_, check_atype_;
+  DBuf<int64_t> check_off_;
+  void fill_jacobian(int block);
+  bool have_jac_structure_ = false;
+  JacobianStructure jac_;
+  DBuf<double> jval_[3];
+  DBuf<int64_t> jstart_;
+  DBuf<int32_t> jsptr_;
   std::vector<DebugRecord> debug_;
   void relinearize(int b);
 

@@ -9,8 +9,12 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <limits>
+#include <string>
 
+#include "./factor_traits.h"
 #include "./geo.h"
 #include "./lm_logic.h"
 #include "./optimizer.h"
@@ -23,6 +27,8 @@ struct LmDevState {
   double err[3];
   double lde;
   double nl2, nu2, dotlu;
+  double zero_diag;        // entries of the damped diagonal below epsilon (CheckHessianDiagonal)
+  double nonfinite_update; // non-finite entries of the update (debug_checks)
   symhip_iteration_stats rec;
   int32_t status, reason, accept, set_best;
 };
@@ -38,6 +44,25 @@ __global__ void lm_reset_kernel(LmDevState* st, double lambda, double nu) {
   st->set_best = 0;
 }
 
+// Key as the reference prints it (opt/key.cc:23-50): letter, then _sub and _super when valid, a
+// negative index written with an n.
+std::string key_string(int8_t letter, int64_t sub, int64_t super) {
+  std::string s(1, (char)letter);
+  for (int64_t v : {sub, super}) {
+    if (v == std::numeric_limits<int64_t>::min()) continue;
+    s += '_';
+    if (v < 0) s += 'n';
+    s += std::to_string(v < 0 ? -v : v);
+  }
+  return s;
+}
+std::string fmt_double(double v) {
+  if (std::isnan(v)) return "nan";
+  char b[32];
+  std::snprintf(b, sizeof(b), "%.6g", v);
+  return b;
+}
+
 // RelaxDampingToInitial (levenberg_marquardt_solver.h:171-174).
 __global__ void lm_relax_kernel(LmDevState* st, double initial_lambda, double beta) {
   st->lambda = fmin(st->lambda, initial_lambda);
@@ -45,6 +70,141 @@ __global__ void lm_relax_kernel(LmDevState* st, double initial_lambda, double be
 }
 
 __global__ void set_value_kernel(double* p, double v) { *p = v; }
+
+// debug_checks: is the factor's linearization finite (CheckLinearizedFactor,
+// opt/internal/linearizer_utils.h:307-367; J finite makes its Hessian and rhs finite)?
+template <int Code>
+__device__ bool factor_finite(const double* const* a) {
+  using F = FT<Code>;
+  double r[F::D];
+  double J[F::D][F::T];
+  F::template eval<double>(a, r, J);
+  bool ok = true;
+  for (int i = 0; i < F::D; ++i) {
+    ok = ok && isfinite(r[i]);
+    for (int c = 0; c < F::T; ++c) ok = ok && isfinite(J[i][c]);
+  }
+  return ok;
+}
+template <int Code>
+__device__ bool factor_finite_upto(int code, const double* const* a) {
+  if constexpr (Code >= kNumFactorTypes) {
+    return true;
+  } else {
+    return code == Code ? factor_finite<Code>(a) : factor_finite_upto<Code + 1>(code, a);
+  }
+}
+// check_derivatives (internal/derivative_checker.h:32-73, NumericalDerivative with delta =
+// sqrt(eps) through the retraction): the factor's analytic Jacobian against central differences
+// of its residual, as three sums of squares (difference, analytic, numerical) for the whole
+// problem's Frobenius-norm comparison.
+template <int Code>
+__device__ void factor_derivative_sums(const double* const* a, const int32_t* atype, double eps,
+                                       double* out) {
+  using F = FT<Code>;
+  double r[F::D], J[F::D][F::T], rp[F::D], rm[F::D], Jx[F::D][F::T];
+  F::template eval<double>(a, r, J);
+  const double delta = sqrt(eps);
+  double sd = 0, sj = 0, sn = 0;
+  for (int li = 0; li < F::NL; ++li) {
+    const int td = F::td(li);
+    const int sdim = atype[li] == SYMHIP_TYPE_POSE3 ? 7 : atype[li] == SYMHIP_TYPE_ROT3 ? 4 : td;
+    for (int k = 0; k < td; ++k) {
+      double vp[8], vm[8], dp[6] = {0, 0, 0, 0, 0, 0}, dm[6] = {0, 0, 0, 0, 0, 0};
+      for (int s = 0; s < sdim; ++s) vp[s] = vm[s] = a[li][s];
+      dp[k] = delta;
+      dm[k] = -delta;
+      retract_value(atype[li], vp, dp, eps);
+      retract_value(atype[li], vm, dm, eps);
+      const double* ap[6];
+      for (int q = 0; q < 6; ++q) ap[q] = a[q];
+      ap[li] = vp;
+      F::template eval<double>(ap, rp, Jx);
+      ap[li] = vm;
+      F::template eval<double>(ap, rm, Jx);
+      const int c = F::foff(li) + k;
+      for (int i = 0; i < F::D; ++i) {
+        const double n = (rp[i] - rm[i]) / (2 * delta);
+        sd += (J[i][c] - n) * (J[i][c] - n);
+        sj += J[i][c] * J[i][c];
+        sn += n * n;
+      }
+    }
+  }
+  out[0] = sd;
+  out[1] = sj;
+  out[2] = sn;
+}
+template <int Code>
+__device__ void factor_derivative_upto(int code, const double* const* a, const int32_t* atype,
+                                       double eps, double* out) {
+  if constexpr (Code < kNumFactorTypes) {
+    if (code == Code) factor_derivative_sums<Code>(a, atype, eps, out);
+    else factor_derivative_upto<Code + 1>(code, a, atype, eps, out);
+  }
+}
+__global__ void factor_derivative_kernel(int32_t n, const int32_t* __restrict__ type,
+                                         const int32_t* __restrict__ ptr,
+                                         const int64_t* __restrict__ off,
+                                         const int32_t* __restrict__ atype,
+                                         const double* __restrict__ values, double eps,
+                                         double* __restrict__ sums) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const double* a[6];
+  const int32_t p0 = ptr[f], na = ptr[f + 1] - p0;
+  for (int k = 0; k < 6; ++k) a[k] = values + off[p0 + (k < na ? k : 0)];
+  factor_derivative_upto<1>(type[f], a, atype + p0, eps, sums + 3 * (int64_t)f);
+}
+
+// include_jacobians: each factor's Jacobian into the sparse Jacobian's values (the columns of
+// its optimized arguments; starts as built by build_jacobian_structure).
+template <int Code>
+__device__ void factor_jacobian_into(const double* const* a, const int64_t* starts,
+                                     double* jval) {
+  using F = FT<Code>;
+  double r[F::D], J[F::D][F::T];
+  F::template eval<double>(a, r, J);
+  for (int t = 0; t < F::T; ++t) {
+    const int64_t q = starts[t];
+    if (q < 0) continue;
+    for (int i = 0; i < F::D; ++i) jval[q + i] = J[i][t];
+  }
+}
+template <int Code>
+__device__ void factor_jacobian_upto(int code, const double* const* a, const int64_t* starts,
+                                     double* jval) {
+  if constexpr (Code < kNumFactorTypes) {
+    if (code == Code) factor_jacobian_into<Code>(a, starts, jval);
+    else factor_jacobian_upto<Code + 1>(code, a, starts, jval);
+  }
+}
+__global__ void jacobian_fill_kernel(int32_t n, const int32_t* __restrict__ type,
+                                     const int32_t* __restrict__ ptr,
+                                     const int64_t* __restrict__ off,
+                                     const int32_t* __restrict__ sptr,
+                                     const int64_t* __restrict__ starts,
+                                     const double* __restrict__ values, double* __restrict__ jval) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const double* a[6];
+  const int32_t p0 = ptr[f], na = ptr[f + 1] - p0;
+  for (int k = 0; k < 6; ++k) a[k] = values + off[p0 + (k < na ? k : 0)];
+  factor_jacobian_upto<1>(type[f], a, starts + sptr[f], jval);
+}
+
+// One thread per factor: bad[f] = 1 when factor f linearizes to a non-finite value at `values`.
+__global__ void factor_check_kernel(int32_t n, const int32_t* __restrict__ type,
+                                    const int32_t* __restrict__ ptr,
+                                    const int64_t* __restrict__ off,
+                                    const double* __restrict__ values, int32_t* __restrict__ bad) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const double* a[6];
+  const int32_t p0 = ptr[f], na = ptr[f + 1] - p0;
+  for (int k = 0; k < 6; ++k) a[k] = values + off[p0 + (k < na ? k : 0)];
+  bad[f] = factor_finite_upto<1>(type[f], a) ? 0 : 1;
+}
 
 // Record the iteration -1 statistics (levenberg_marquardt_solver.tcc:158-186).
 __global__ void lm_first_record_kernel(LmDevState* st, int init) {
@@ -81,20 +241,40 @@ __global__ void retract_kernel(int32_t n, const int32_t* __restrict__ type,
   retract_value(type[i], values + soff[i], update + toff[i], eps);
 }
 
-// sum_i u_i (rhs_i - damping_i u_i)  (Linearization::LinearDeltaError, linearization.h:62-67)
+// sum_i u_i (rhs_i - damping_i u_i)  (Linearization::LinearDeltaError, linearization.h:62-67),
+// and in the same pass the two counts of the iteration's checks: entries of the damped diagonal
+// (undamped + damping) below eps in magnitude (CheckHessianDiagonal, lm.tcc:56-90) and
+// non-finite entries of the update (lm.tcc:225-227). Partials: part[b], part[nb + b],
+// part[2 nb + b].
 __global__ void __launch_bounds__(256) lde_partial_kernel(const double* __restrict__ u,
                                                           const double* __restrict__ rhs,
                                                           const double* __restrict__ damp,
-                                                          int64_t n, double* __restrict__ part) {
-  __shared__ double ws[4];
-  double acc = 0;
+                                                          const double* __restrict__ und,
+                                                          double eps, int64_t n,
+                                                          double* __restrict__ part) {
+  __shared__ double ws[3][4];
+  double acc = 0, zeros = 0, bad = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    acc += u[i] * (rhs[i] - damp[i] * u[i]);
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double ui = u[i], di = damp[i];
+    acc += ui * (rhs[i] - di * ui);
+    zeros += fabs(und[i] + di) < eps ? 1.0 : 0.0;
+    bad += isfinite(ui) ? 0.0 : 1.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_down(acc, o, 64);
+    zeros += __shfl_down(zeros, o, 64);
+    bad += __shfl_down(bad, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ws[0][threadIdx.x >> 6] = acc;
+    ws[1][threadIdx.x >> 6] = zeros;
+    ws[2][threadIdx.x >> 6] = bad;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+  if (threadIdx.x < 3)
+    part[threadIdx.x * gridDim.x + blockIdx.x] =
+        ws[threadIdx.x][0] + ws[threadIdx.x][1] + ws[threadIdx.x][2] + ws[threadIdx.x][3];
 }
 __global__ void __launch_bounds__(256) sum_into_kernel(const double* __restrict__ part, int nb,
                                                        double* __restrict__ out) {
@@ -206,6 +386,7 @@ class CscBackend final : public SystemBackend {
   const double* rhs(int b) const override { return rhs_[b].get(); }
   const double* residual(int b) const override { return residual_[b].get(); }
   const double* damping() const override { return damping_.get(); }
+  const double* undamped_diagonal() const override { return undamped_.get(); }
   void hessian_csc(int b, double* out, hipStream_t s) override {
     hval_[b].download(out, nnz_, s);
   }
@@ -254,6 +435,9 @@ std::unique_ptr<SystemBackend> make_csc_backend(const HostProblem& p, const Solv
 DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_optimizer_params& p,
                                  const SolverOptions& o, double epsilon)
     : p_(p), o_(o), epsilon_(epsilon) {
+  // Optimizer ctor (optimizer.tcc:39): the derivative check compares against the Jacobian
+  SYMHIP_CHECK(!p.check_derivatives || p.include_jacobians,
+               "check_derivatives requires include_jacobians");
   SYMHIP_CHECK(o.precision == SYMHIP_PRECISION_F64 ||
                    (o.precision == SYMHIP_PRECISION_F32_JAC && o.linear_solver == kSolverSchur),
                "symhip: SYMHIP_PRECISION_F32_JAC is available on the Schur path only");
@@ -267,7 +451,7 @@ DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_opti
   for (auto& v : values_) v.resize(std::max<int64_t>(1, host_.storage_total));
   update_.resize(std::max(1, host_.N));
   last_update_.resize(std::max(1, host_.N));
-  partials_.resize(kReduceBlocks);
+  partials_.resize(3 * kReduceBlocks);
   std::vector<int32_t> rt, rs, rto;
   for (int32_t k : host_.state_keys) {
     rt.push_back(host_.types[k]);
@@ -311,9 +495,194 @@ void DeviceOptimizer::best_linearization(double* residual, double* hval, double*
 }
 
 void DeviceOptimizer::relinearize(int b) {
-  PhaseScope ps(&timer_, SYMHIP_PHASE_LINEARIZE, stream_);
-  backend_->relinearize(b, values_[b].get(), &dstate_->err[b], stream_);
+  {
+    PhaseScope ps(&timer_, SYMHIP_PHASE_LINEARIZE, stream_);
+    backend_->relinearize(b, values_[b].get(), &dstate_->err[b], stream_);
+  }
   linearized_[b] = true;
+  if (p_.include_jacobians) fill_jacobian(b);
+  if (p_.debug_checks) check_factors(b);
+  if (p_.check_derivatives) check_derivatives(b);
+}
+
+// The iteration's checks, from the counts the linear-error pass left in the device state:
+// CheckHessianDiagonal (lm.tcc:56-90; the indices are found on the host, only when there are
+// any), the non-finite update under debug_checks (:225-227) and the non-finite new error
+// (:257-262).
+void DeviceOptimizer::check_iteration() {
+  const symhip_iteration_stats& rec = hstate_->rec;
+  if (p_.verbose) {
+    // PopulateIterationStats under verbose (lm.tcc:105-113)
+    const double prev = hstate_->err[init_];
+    const double gain = (prev - rec.new_error) / (prev - rec.new_error_linear);
+    std::fprintf(stderr,
+                 "[symhip] LM<symhip> [iter %4d] lambda: %.3e, error prev/linear/new: "
+                 "%.3e/%.3e/%.3e, rel reduction: %.5e, gain ratio: %.5e\n",
+                 rec.iteration, rec.current_lambda, prev, rec.new_error_linear, rec.new_error,
+                 rec.relative_reduction, gain);
+  }
+  if (hstate_->zero_diag > 0) {
+    const int32_t N = host_.N;
+    std::vector<double> u(N), d(N);
+    HIP_OK(hipMemcpyAsync(u.data(), backend_->undamped_diagonal(), sizeof(double) * N,
+                          hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipMemcpyAsync(d.data(), backend_->damping(), sizeof(double) * N,
+                          hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    std::vector<int32_t> idx;
+    for (int32_t i = 0; i < N; ++i)
+      if (std::fabs(u[i] + d[i]) < epsilon_) idx.push_back(i);
+    char head[160];
+    std::snprintf(head, sizeof(head),
+                  "LM<symhip> Zero on diagonal after damping (with lambda = %.2e, epsilon = "
+                  "%.2e) at indices: ",
+                  rec.current_lambda, epsilon_);
+    constexpr size_t kShow = 15;
+    std::string m = std::string(head) + "[";
+    for (size_t i = 0; i < std::min(kShow, idx.size()); ++i)
+      m += (i ? ", " : "") + std::to_string(idx[i]);
+    if (idx.size() > kShow) m += ", ... (" + std::to_string(idx.size() - kShow) + " omitted)";
+    warn(m + "]");
+  }
+  if (p_.debug_checks && hstate_->nonfinite_update > 0)
+    warn("LM<symhip> Non-finite update: " + std::to_string((int64_t)hstate_->nonfinite_update) +
+         " of " + std::to_string(host_.N) + " entries");
+  if (!std::isfinite(rec.new_error)) {
+    warn("LM<symhip> Encountered non-finite error: " + fmt_double(rec.new_error));
+    if (!p_.debug_checks)
+      warn("LM<symhip> Turn on debug_checks to see which factor is causing this");
+  }
+}
+
+void DeviceOptimizer::warn(const std::string& msg) {
+  std::fprintf(stderr, "[symhip] %s\n", msg.c_str());
+  warnings_.push_back(msg);
+}
+
+const JacobianStructure& DeviceOptimizer::jacobian_structure() {
+  if (!have_jac_structure_) {
+    build_jacobian_structure(host_, jac_);
+    have_jac_structure_ = true;
+  }
+  return jac_;
+}
+
+void DeviceOptimizer::fill_jacobian(int b) {
+  const int32_t nf = host_.n_factors;
+  if (nf == 0) return;
+  upload_check_tables();
+  if (jstart_.size() == 0) {
+    const JacobianStructure& j = jacobian_structure();
+    jstart_.upload(j.starts, stream_);
+    jsptr_.upload(j.sptr, stream_);
+    for (auto& v : jval_) v.resize(std::max<size_t>(1, j.rowind.size()));
+  }
+  hipLaunchKernelGGL(jacobian_fill_kernel, dim3(grid_for(nf, 64)), dim3(64), 0, stream_, nf,
+                     check_type_.get(), check_ptr_.get(), check_off_.get(), jsptr_.get(),
+                     jstart_.get(), values_[b].get(), jval_[b].get());
+}
+
+void DeviceOptimizer::best_jacobian(double* values) {
+  SYMHIP_CHECK(p_.include_jacobians && jstart_.size() != 0,
+               "no Jacobian: create the optimizer with include_jacobians and optimize first");
+  jval_[best_].download(values, jac_.rowind.size(), stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void DeviceOptimizer::upload_check_tables() {
+  if (check_type_.size() != 0) return;
+  std::vector<int64_t> off;
+  std::vector<int32_t> at;
+  for (int32_t k : host_.fargs) {
+    off.push_back(host_.storage_offset[k]);
+    at.push_back(host_.types[k]);
+  }
+  check_type_.upload(host_.ftype, stream_);
+  check_ptr_.upload(host_.fargs_ptr, stream_);
+  check_off_.upload(off, stream_);
+  check_atype_.upload(at, stream_);
+  check_bad_.resize(std::max(1, host_.n_factors));
+}
+
+// check_derivatives: the Jacobian of the whole problem at block b's Values against its central
+// differences, Eigen's isApprox with precision 10 sqrt(eps) (|J - Jn| <= p min(|J|, |Jn|),
+// Frobenius norms; derivative_checker.h:53-58), a failure reported with the worst factors as
+// the reference's assertion fails the optimization (optimizer.tcc:261-270).
+void DeviceOptimizer::check_derivatives(int b) {
+  const int32_t nf = host_.n_factors;
+  if (nf == 0) return;
+  upload_check_tables();
+  DBuf<double> sums;
+  sums.resize((size_t)3 * nf);
+  hipLaunchKernelGGL(factor_derivative_kernel, dim3(grid_for(nf, 64)), dim3(64), 0, stream_, nf,
+                     check_type_.get(), check_ptr_.get(), check_off_.get(), check_atype_.get(),
+                     values_[b].get(), epsilon_, sums.get());
+  std::vector<double> s((size_t)3 * nf);
+  sums.download(s.data(), s.size(), stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+  double sd = 0, sj = 0, sn = 0;
+  for (int32_t f = 0; f < nf; ++f) {
+    sd += s[3 * f];
+    sj += s[3 * f + 1];
+    sn += s[3 * f + 2];
+  }
+  const double prec = 10 * std::sqrt(epsilon_);
+  if (std::sqrt(sd) <= prec * std::min(std::sqrt(sj), std::sqrt(sn))) return;
+  std::vector<int32_t> worst(nf);
+  for (int32_t f = 0; f < nf; ++f) worst[f] = f;
+  const size_t k = std::min<size_t>(5, worst.size());
+  std::partial_sort(worst.begin(), worst.begin() + k, worst.end(),
+                    [&](int32_t x, int32_t y) { return s[3 * x] > s[3 * y]; });
+  std::string m = "check_derivatives: symbolic and numerical jacobians don't match (|J - Jn| = " +
+                  fmt_double(std::sqrt(sd)) + ", |J| = " + fmt_double(std::sqrt(sj)) +
+                  ", |Jn| = " + fmt_double(std::sqrt(sn)) + "); largest differences in factors";
+  for (size_t i = 0; i < k; ++i)
+    m += " " + std::to_string(worst[i]) + " (" + fmt_double(std::sqrt(s[3 * worst[i]])) + ")";
+  throw Error(m);
+}
+
+// debug_checks: the factors whose linearization at block b's Values is not finite, reported as
+// the reference's linearizer reports them (opt/internal/linearizer_utils.h:307-367): the factor,
+// each argument's key, offset, size and value, and its residual.
+void DeviceOptimizer::check_factors(int b) {
+  const int32_t nf = host_.n_factors;
+  if (nf == 0) return;
+  upload_check_tables();
+  hipLaunchKernelGGL(factor_check_kernel, dim3(grid_for(nf, 128)), dim3(128), 0, stream_, nf,
+                     check_type_.get(), check_ptr_.get(), check_off_.get(), values_[b].get(),
+                     check_bad_.get());
+  std::vector<int32_t> bad(nf);
+  check_bad_.download(bad.data(), nf, stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+  std::vector<double> vals, res;
+  for (int32_t f = 0; f < nf; ++f) {
+    if (!bad[f]) continue;
+    if (vals.empty()) {
+      vals.resize(host_.storage_total);
+      res.resize(host_.M);
+      values_[b].download(vals.data(), vals.size(), stream_);
+      if (host_.M)
+        HIP_OK(hipMemcpyAsync(res.data(), backend_->residual(b), sizeof(double) * host_.M,
+                              hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+    }
+    const FactorSpec sp = factor_spec(host_.ftype[f]);
+    std::string m = "LM<symhip> Non-finite linearization for factor " + std::to_string(f) +
+                    " (type " + std::to_string(host_.ftype[f]) + "):";
+    for (int32_t q = host_.fargs_ptr[f]; q < host_.fargs_ptr[f + 1]; ++q) {
+      const int32_t k = host_.fargs[q];
+      const int32_t o = host_.storage_offset[k], w = storage_dim(host_.types[k]);
+      m += "\n  " + key_string(host_.letters[k], host_.subs[k], host_.supers[k]) +
+           " (offset=" + std::to_string(o) + ", size=" + std::to_string(w) + ") = [";
+      for (int32_t i = 0; i < w; ++i) m += (i ? ", " : "") + fmt_double(vals[o + i]);
+      m += "]";
+    }
+    m += "\nResidual: [";
+    for (int i = 0; i < sp.res_dim; ++i)
+      m += (i ? ", " : "") + fmt_double(res[host_.res_off[f] + i]);
+    m += "]";
+    warn(m);
+  }
 }
 
 void DeviceOptimizer::optimize(double* values_inout, int32_t num_iterations, bool populate_best,
@@ -348,6 +717,7 @@ void DeviceOptimizer::reset_run() {
                      p_.dynamic_lambda_update_beta);
   backend_->begin_run();
   debug_.clear();
+  warnings_.clear();
   iter_times_.clear();
   run_start_ = std::chrono::steady_clock::now();
 }
@@ -466,6 +836,10 @@ void DeviceOptimizer::record_debug(int block, bool with_update) {
     r.update.resize(host_.N);
     update_.download(r.update.data(), r.update.size(), stream_);
   }
+  if (p_.include_jacobians && jstart_.size() != 0) {
+    r.jacobian.resize(jac_.rowind.size());
+    jval_[block].download(r.jacobian.data(), r.jacobian.size(), stream_);
+  }
   HIP_OK(hipStreamSynchronize(stream_));
   debug_.push_back(std::move(r));
 }
@@ -488,6 +862,12 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
     HIP_OK(hipStreamSynchronize(stream_));
     iters.push_back(hstate_->rec);
     if (p_.debug_stats) record_debug(init_, false);
+    if (!std::isfinite(hstate_->rec.new_error)) {
+      warn("LM<symhip> Encountered non-finite initial error: " +
+           fmt_double(hstate_->rec.new_error));
+      if (!p_.debug_checks)
+        warn("LM<symhip> Turn on debug_checks to see which factor is causing this");
+    }
     if (hstate_->status == SYMHIP_STATUS_FAILED) {
       status = hstate_->status;
       reason = hstate_->reason;
@@ -519,9 +899,14 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   {
     const int nb = std::max(1, std::min(kReduceBlocks, grid_for(N, 256)));
     hipLaunchKernelGGL(lde_partial_kernel, dim3(nb), dim3(256), 0, stream_, update_.get(),
-                       backend_->rhs(init_), backend_->damping(), (int64_t)N, partials_.get());
+                       backend_->rhs(init_), backend_->damping(), backend_->undamped_diagonal(),
+                       epsilon_, (int64_t)N, partials_.get());
     hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, stream_, partials_.get(), nb,
                        &dstate_->lde);
+    hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, stream_, partials_.get() + nb, nb,
+                       &dstate_->zero_diag);
+    hipLaunchKernelGGL(sum_into_kernel, dim3(1), dim3(256), 0, stream_,
+                       partials_.get() + 2 * nb, nb, &dstate_->nonfinite_update);
   }
   if (p_.enable_bold_updates && have_last_update_) {
     dot(last_update_.get(), last_update_.get(), N, partials_.get(), &dstate_->nl2, stream_);
@@ -536,6 +921,7 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   timer_.collect();
   HIP_OK(hipGetLastError());
   iters.push_back(hstate_->rec);
+  check_iteration();
   if (p_.debug_stats) record_debug(new_, true);
   backend_->observe(hstate_->rec);
   if (!hstate_->accept) {

@@ -190,4 +190,49 @@ void build_host_problem(const symhip_problem_desc* d, HostProblem& p) {
   p.cs_ptr[p.n_factors] = (int32_t)p.col_starts.size();
 }
 
+void build_jacobian_structure(const HostProblem& p, JacobianStructure& j) {
+  // rows of column c: the residual rows of every factor whose optimized arguments cover c, in
+  // factor order (= row order)
+  std::vector<int64_t> count(p.N, 0);
+  auto for_columns = [&](int32_t f, auto&& fn) {
+    const FactorSpec s = factor_spec(p.ftype[f]);
+    int32_t t = 0;
+    for (int a = 0; a < s.nargs; ++a) {
+      if (!(s.lin_mask & (1 << a))) continue;
+      const int32_t k = p.fargs[p.fargs_ptr[f] + a];
+      const int32_t td = tangent_dim(p.types[k]);
+      for (int32_t c = 0; c < td; ++c) fn(t + c, p.tan_off[k] < 0 ? -1 : p.tan_off[k] + c);
+      t += td;
+    }
+  };
+  for (int32_t f = 0; f < p.n_factors; ++f) {
+    const int32_t D = factor_spec(p.ftype[f]).res_dim;
+    for_columns(f, [&](int32_t, int32_t col) {
+      if (col >= 0) count[col] += D;
+    });
+  }
+  j.colptr.assign(p.N + 1, 0);
+  for (int32_t c = 0; c < p.N; ++c) {
+    j.colptr[c + 1] = (int32_t)(j.colptr[c] + count[c]);
+    SYMHIP_CHECK(j.colptr[c + 1] >= j.colptr[c], "jacobian exceeds 2^31 nonzeros");
+  }
+  j.rowind.assign(j.colptr[p.N], 0);
+  std::vector<int64_t> fill(j.colptr.begin(), j.colptr.end() - 1);
+  j.sptr.assign(p.n_factors + 1, 0);
+  j.starts.clear();
+  for (int32_t f = 0; f < p.n_factors; ++f) {
+    const FactorSpec s = factor_spec(p.ftype[f]);
+    j.sptr[f] = (int32_t)j.starts.size();
+    j.starts.resize(j.starts.size() + s.tdim, -1);
+    for_columns(f, [&](int32_t t, int32_t col) {
+      if (col < 0) return;
+      const int64_t q = fill[col];
+      j.starts[j.sptr[f] + t] = q;
+      for (int32_t r = 0; r < s.res_dim; ++r) j.rowind[q + r] = p.res_off[f] + r;
+      fill[col] += s.res_dim;
+    });
+  }
+  j.sptr[p.n_factors] = (int32_t)j.starts.size();
+}
+
 }  // namespace symhip

@@ -42,4 +42,16 @@ struct HostProblem {
 // Builds everything above from the boundary description; throws symhip::Error on bad input.
 void build_host_problem(const symhip_problem_desc* d, HostProblem& p);
 
+// The sparse Jacobian (M x N, CSC, rows sorted) of the reference's linearizer under
+// include_jacobians (linearizer.cc:475-538 triplets, one D x td block per factor and optimized
+// key), and where each factor's columns land in it: for factor f and column t of its tangent
+// layout (linearized arguments in order, T = FactorSpec::tdim columns), starts[sptr[f] + t] is
+// the position of row res_off[f] of that column in the value array (its D rows follow), or -1
+// when the argument is not optimized.
+struct JacobianStructure {
+  std::vector<int32_t> colptr, rowind, sptr;
+  std::vector<int64_t> starts;
+};
+void build_jacobian_structure(const HostProblem& p, JacobianStructure& j);
+
 }  // namespace symhip

@@ -270,7 +270,7 @@ __global__ void lin_bfac_kernel(int32_t n, const int32_t* __restrict__ arg_off,
 }
 
 // Damping of camera and point diagonals (DampHessian semantics, tcc:22-54), writing the damping
-// vector in reference state order.
+// vector and the undamped diagonal in reference state order.
 template <int ND>
 __global__ void damp_schur_kernel(int32_t n_nodes, int32_t n_points,
                                   const double* __restrict__ Bdiag, const double* __restrict__ Bfac,
@@ -279,8 +279,8 @@ __global__ void damp_schur_kernel(int32_t n_nodes, int32_t n_points,
                                   const int32_t* __restrict__ pt_tan,
                                   const double* __restrict__ lambda_p, int use_diag, int keep_max,
                                   int have_max, double dmin, int unit, double* __restrict__ max_diag,
-                                  double* __restrict__ damping, double* __restrict__ dnode,
-                                  double* __restrict__ dpoint) {
+                                  double* __restrict__ damping, double* __restrict__ undamped,
+                                  double* __restrict__ dnode, double* __restrict__ dpoint) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nb = n_nodes * ND;
   if (i >= nb + 3 * n_points) return;
@@ -309,6 +309,7 @@ __global__ void damp_schur_kernel(int32_t n_nodes, int32_t n_points,
   }
   if (unit) d += lambda;
   damping[tan] = d;
+  undamped[tan] = u;
   if (i < nb) dnode[i] = d; else dpoint[i - nb] = d;
 }
 
@@ -728,6 +729,7 @@ class SchurBackend final : public SystemBackend {
   const double* rhs(int b) const override { return rhs_[b].get(); }
   const double* residual(int b) const override { return residual_[b].get(); }
   const double* damping() const override { return damping_.get(); }
+  const double* undamped_diagonal() const override { return undamped_.get(); }
   void hessian_csc(int b, double* out, hipStream_t s) override;
   int32_t N() const override { return N_; }
   int32_t M() const override { return M_; }
@@ -799,7 +801,8 @@ class SchurBackend final : public SystemBackend {
   std::vector<BGroup> bgroups_;
   // per state block storage
   DBuf<double> residual_[3], W_[3], V_[3], gp_[3], Bdiag_[3], Bfac_[3], gB_[3], rhs_[3];
-  DBuf<double> damping_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, ZZ_, v_, y_, partials_;
+  DBuf<double> damping_, undamped_, max_diag_, dnode_, dpoint_, Lp_, hp_, Z_, ZZ_, v_, y_,
+      partials_;
   // FP32 form (precision 1): while PCG runs, W and Z are stored in FP32 (W32_, Z32_) and so is S
   // for PCG; residual, gradient, sums, LDL^T and the LM decisions stay FP64. Once the run has
   // switched to the exact LDL^T (exact_now_), linearization returns to plain FP64 storage, the
@@ -1285,6 +1288,7 @@ SchurBackend::SchurBackend(const HostProblem& P, const SolverOptions& o, hipStre
     rhs_[b].resize(std::max(1, N_));
   }
   damping_.resize(std::max(1, N_));
+  undamped_.resize(std::max(1, N_));
   max_diag_.resize(std::max(1, N_));
   dnode_.resize((size_t)n_nodes_ * nd_);
   dpoint_.resize((size_t)n_points_ * 3);
@@ -1361,8 +1365,8 @@ void SchurBackend::damp(int b, const double* lambda_dev, const symhip_optimizer_
                        Bdiag_[b].get(), Bfac_[b].get(), diag_blk_.get(), V_[b].get(),
                        node_tan_.get(), pt_tan_.get(), lambda_dev, p.use_diagonal_damping,
                        p.keep_max_diagonal_damping, have_max ? 1 : 0, p.diagonal_damping_min,
-                       p.use_unit_damping, max_diag_.get(), damping_.get(), dnode_.get(),
-                       dpoint_.get());
+                       p.use_unit_damping, max_diag_.get(), damping_.get(), undamped_.get(),
+                       dnode_.get(), dpoint_.get());
   };
   if (nd_ == 9) launch(damp_schur_kernel<9>); else launch(damp_schur_kernel<6>);
 }
