@@ -13,13 +13,15 @@ struct FactorGroup {
   int32_t n = 0;
   DBuf<int32_t> arg_off;  // [NA][n] storage offsets of each argument (SoA over factors)
   DBuf<int32_t> res_off;  // [n]
-  DBuf<int32_t> tan_off;  // [NL][n] tangent offsets of linearized args (-1: not optimized)
-  DBuf<int32_t> cs_ptr;   // [n] first col-start of the factor
+  int64_t contrib_off = 0;  // first entry of the group's local outputs in the contribution buffer
 };
 
-// Linearizer::Relinearize (opt/linearizer.cc:57-112) on the device: zero rhs/H, evaluate all
-// factors of each type in one launch, scatter residual / rhs / H through the reference's storage
-// maps with FP64 atomics.
+// Linearizer::Relinearize (opt/linearizer.cc:57-112) on the device, without atomics. Pass 1, one
+// thread per factor and a launch per factor type: the residual into its rows and the factor's
+// local rhs J^T r and Hessian blocks J^T J (lower diagonal blocks, full off-diagonal blocks) into
+// a per-factor contribution record. Pass 2, one thread per rhs / hessian_lower entry: the sum of
+// its contributions, gathered through a host-built list in factor order (the order in which the
+// reference's UpdateFromLinearizedDenseFactorIntoSparse adds them, linearizer.cc:358-433).
 class CscLinearizer {
  public:
   void setup(const HostProblem& p, hipStream_t s);
@@ -30,7 +32,8 @@ class CscLinearizer {
 
  private:
   std::vector<FactorGroup> groups_;
-  DBuf<int32_t> col_starts_;
+  DBuf<double> contrib_;
+  DBuf<int32_t> hptr_, hidx_, rptr_, ridx_;  // gather lists: CSR over hval and rhs entries
 };
 
 }  // namespace symhip
