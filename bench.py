@@ -18,6 +18,11 @@ Prints one JSON line (value = LM iterations/s of the timed FP64 run) with:
                 same problem after one untimed iteration (setup and the one-time symbolic
                 analysis excluded), the reduced camera system ordered by nested dissection as the
                 reference orders it with METIS; also the -ffast-math build of the same sources
+  batch         4096 robot-3D-localization problems in one persistent launch (FP64 and FP32
+                Jacobian forms), against the CPU oracle on one core and on all cores
+  pose_graph    the 100 000-node pose graph (Cholesky path): LM iterations/s, phases, run to
+                convergence, the factorization phase's roofline, and the CPU oracle on the same
+                problem and elimination order
 """
 from __future__ import annotations
 
@@ -277,6 +282,70 @@ def batch_leg(n_problems=4096, cpu_sample=64):
     return out
 
 
+def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
+    """Pose-graph SLAM at 100 000 Pose3 nodes (115 037 between factors, 15 037 of them Cauchy
+    loop closures): the sparse Cholesky path (supernodal LDL^T over the nested-dissection
+    order) timed for `steps` LM iterations with phase timing, the factorization phase against
+    the FP64 matrix peak, a run to convergence, and the CPU oracle (restated reference LM +
+    SparseCholeskySolver, the same elimination order) for `cpu_iters` steady iterations."""
+    prob = capi.gen_pose_graph(100000, 30000, 42)
+    opts = capi.solver_options(linear_solver=capi.SOLVER_CHOLESKY,
+                               ordering=capi.ORDERING_NESTED_DISSECTION)
+    opt = capi.HipOptimizer(prob, lm_params(steps + 2, early_exit=0.0), opts)
+    info = opt.solver_info()
+    opt.set_values(prob.values)
+    opt.optimize_resident(num_iterations=2)
+    t0 = time.perf_counter()
+    opt.optimize_resident(num_iterations=steps)
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    opt.enable_timing(True)
+    opt.optimize_resident(num_iterations=steps)
+    phases = {k: v[0] / max(1, v[1]) for k, v in opt.phase_times().items() if v[1]}
+    opt.enable_timing(False)
+    conv = capi.HipOptimizer(prob, capi.default_params(iterations=100), opts)
+    conv.set_values(prob.values)
+    conv.optimize_resident()  # warm
+    t0 = time.perf_counter()
+    res = conv.optimize_resident()
+    t_conv = time.perf_counter() - t0
+    flops = float(info["flops"])
+    out = {"workload": "pose_graph_100k_nodes_115k_edges_cauchy_loops",
+           "n_loops": int(prob.info["n_loops"]), "unknowns": int(info["n"]),
+           "lm_iters_per_s": 1e3 / ms, "ms_per_iter": ms, "phases_ms_per_iter": phases,
+           "converge": {"s": t_conv, "iterations": len(res["iterations"]) - 1,
+                        "status": res["status"],
+                        "final_cost": res["iterations"][res["best_index"]]["new_error"]},
+           "factor_roofline": {"kernel": "supernodal LDL^T factorization phase",
+                               "bound": "mfma", "levels": int(info["levels"]),
+                               "supernodes": int(info["supernodes"]),
+                               "algorithmic_flops": flops,
+                               "achieved": flops / (phases["factor"] * 1e-3) / 1e12,
+                               "peak": 78.6, "unit": "TFLOP/s",
+                               "frac": flops / (phases["factor"] * 1e-3) / 1e12 / 78.6}}
+    if not cpu:
+        return out
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_wrap as ow
+    hp = capi.HipProblem(prob)
+    cp, ri = hp.hessian_structure()
+    _, offsets = hp.state_keys()
+    node_ptr = np.append(offsets, len(cp) - 1).astype(np.int32)
+    order = capi.HipCholesky(len(cp) - 1, cp, ri, node_ptr=node_ptr,
+                             ordering=capi.ORDERING_NESTED_DISSECTION).inv_permutation()
+    op = ow.OracleProblem(prob)
+    secs, _ = ow.time_iterations(op, lm_params(cpu_iters + 1, early_exit=0.0), cpu_iters + 1,
+                                 solver=ow.CHOLESKY, ordering=order)
+    model, _ = host_cpu()
+    out["cpu_baseline"] = {"value": 1.0 / float(np.mean(secs[1:])), "unit": "LM iters/s",
+                           "cores": 1, "kind": "port", "cpu": model,
+                           "sample": (f"CPU FP64 oracle (restated reference LM + "
+                                      f"SparseCholeskySolver, the same nested-dissection order), "
+                                      f"same problem: {cpu_iters} steady LM iterations after one "
+                                      f"untimed one ({secs[0]:.1f} s with the symbolic "
+                                      f"analysis): {np.mean(secs[1:]):.2f} s/iter")}
+    return out
+
+
 def schur_elimination_order(prob):
     """Elimination order of the reduced camera system for the CPU oracle: nested dissection of
     the camera covisibility graph (the reference orders S with METIS, sparse_schur_solver.h:77),
@@ -362,6 +431,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
+    ap.add_argument("--no-pose-graph", action="store_true",
+                    help="skip the 100k-node pose-graph leg")
+    ap.add_argument("--pose-graph-only", action="store_true",
+                    help="only run the pose-graph leg's GPU part and print it (for profiling)")
     ap.add_argument("--fast-precision", choices=("f32", "f64"), default="f32",
                     help="storage precision of the fast form's W, Z and S")
     ap.add_argument("--pcg-tol", type=float, default=3e-2)
@@ -378,6 +451,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.pose_graph_only:
+        print(json.dumps(pose_graph_leg(cpu=False)))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the optimizer path runs on a single GPU "
@@ -425,6 +501,8 @@ def main():
     }
     if not args.no_batch:
         out["batch"] = batch_leg()
+    if not args.no_pose_graph:
+        out["pose_graph"] = pose_graph_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(r["prob"])
     print(json.dumps(out))

@@ -201,9 +201,14 @@ __device__ __forceinline__ void diag_body(const SnTask& tk, double* __restrict__
     for (int r = 0; r < 4; ++r) acc[c][r] = Lc[16 * c + lr][16 * wave + lk + 4 * r];
   // (the first barrier of the sweep orders these reads before the owners' Lc writes)
   const bool below_wave = wave >= kMaxCols / 16;
+  // Steps past the supernode's last column are skipped (the padding stays the identity, which
+  // its elimination would not change): a 6-column supernode takes 2 steps, not 16. (Skipping the
+  // padding's MFMA tiles, or ending the solve chains at n, measured slower: the extra runtime
+  // conditions cost the unrolled loops their schedule.)
 #pragma unroll
   for (int st = 0; st < kMaxCols / kStep; ++st) {
     const int k0 = kStep * st;
+    if (k0 >= n) break;
     {
       // publish columns k0 .. k0 + 7 (tile column k0 / 16, lanes k0 % 16 ..) of the live rows
       const int off = k0 & 15, cp = k0 >> 4;
