@@ -247,6 +247,7 @@ def batch_leg(n_problems=4096, cpu_sample=64):
     for name, prec in (("f64", capi.PRECISION_F64), ("f32_jac", capi.PRECISION_F32_JAC)):
         b = capi.HipBatch(probs, params, precision=prec)
         b.solve(records=False)  # warm-up
+        chosen, config_ms = b.tune(reps=2)
         runs = [b.solve(records=False) for _ in range(3)]
         r = min(runs, key=lambda x: x["total_ms"])
         ok = sum(p["status"] == capi.STATUS_SUCCESS for p in r["problems"])
@@ -254,7 +255,10 @@ def batch_leg(n_problems=4096, cpu_sample=64):
                      "launch_ms": r["total_ms"],
                      "p50_solve_ms": float(np.percentile(r["solve_ms"], 50)),
                      "p99_solve_ms": float(np.percentile(r["solve_ms"], 99)),
-                     "converged": ok}
+                     "converged": ok,
+                     "autotune": {"chosen": capi.HipBatch.BATCH_CONFIGS[chosen],
+                                  "config_ms": {capi.HipBatch.BATCH_CONFIGS[c]: float(t)
+                                                for c, t in enumerate(config_ms)}}}
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
     from concurrent.futures import ThreadPoolExecutor

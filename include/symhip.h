@@ -79,6 +79,18 @@ int symhip_batch_solve(symhip_batch_t batch, double* values_out,
                        symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
                        double* solve_ms, double* total_ms);
 
+/* Autotuner of the batched solve (no counterpart in the reference): solves the whole batch
+   `reps` times in each of the 4 launch configurations (bit 0: the kernel built for two
+   wavefronts per SIMD instead of one, i.e. twice the resident problems for ~120 spilled
+   registers; bit 1: problems with the most poses launched first) and keeps the fastest for
+   later symhip_batch_solve calls. config_ms (4 entries, may be NULL) receives each one's best
+   launch time, *chosen (may be NULL) the configuration kept. The results of every
+   configuration are the same. */
+int symhip_batch_tune(symhip_batch_t batch, int32_t reps, double* config_ms, int32_t* chosen);
+/* Selects launch configuration `config` (0-3, as above) for later solves, e.g. one a previous
+   tuning run of the same batch shape chose. */
+int symhip_batch_set_config(symhip_batch_t batch, int32_t config);
+
 /* Allocation-free variant of symhip_batch_create + symhip_batch_solve. No counterpart in the
    reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
    serving loops, harnesses that forbid hipMalloc). The caller packs the batch on the host with

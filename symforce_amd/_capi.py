@@ -453,6 +453,8 @@ _EXTRA_SIGNATURES: list = [
     ("symhip_batch_storage_size", [_vp], C.c_int64),
     ("symhip_batch_solve", [_vp, F64P, C.POINTER(OptimizationStats), C.POINTER(IterationStats),
                             F64P, F64P], C.c_int),
+    ("symhip_batch_tune", [_vp, C.c_int32, F64P, I32P], C.c_int),
+    ("symhip_batch_set_config", [_vp, C.c_int32], C.c_int),
     ("symhip_batch_pack_sizes", [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(OptimizerParams),
                                  I64P], C.c_int),
     ("symhip_batch_pack", [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(OptimizerParams), I32P,
@@ -741,6 +743,21 @@ class HipBatch:
         if getattr(self, "h", None):
             hip_lib().symhip_batch_destroy(self.h)
             self.h = None
+
+    BATCH_CONFIGS = ("1 wave/SIMD, given order", "2 waves/SIMD, given order",
+                     "1 wave/SIMD, most poses first", "2 waves/SIMD, most poses first")
+
+    def tune(self, reps=2):
+        """The batch autotuner (symhip_batch_tune): times each launch configuration over `reps`
+        solves, keeps the fastest; returns (chosen index, per-configuration ms)."""
+        ms = np.zeros(4)
+        chosen = np.zeros(1, np.int32)
+        check(hip_lib().symhip_batch_tune(self.h, reps, ptr(ms, C.c_double),
+                                          ptr(chosen, C.c_int32)))
+        return int(chosen[0]), ms
+
+    def set_config(self, config):
+        check(hip_lib().symhip_batch_set_config(self.h, config))
 
     def solve(self, records=True):
         n = len(self.problems)

@@ -46,6 +46,17 @@ class BatchLocalization {
   // solve_ms the per-problem wall time inside the launch.
   void solve(double* values_out, symhip_optimization_stats* stats_out,
              symhip_iteration_stats* iters_out, double* solve_ms, double* total_ms);
+  // Times every launch configuration (occupancy: one or two wavefronts per SIMD; problem order:
+  // as given or most poses first) over `reps` complete solves each and keeps the fastest for
+  // later solves; config_ms (kConfigs entries, may be null) receives each one's best time.
+  static constexpr int kConfigs = 4;
+  void tune(int reps, double* config_ms);
+  int config() const { return config_; }
+  void set_config(int c) {
+    SYMHIP_CHECK(c >= 0 && c < kConfigs, "batch: launch configuration out of range");
+    build_graph(c);
+    config_ = c;
+  }
   int32_t n_problems() const { return n_problems_; }
   int64_t storage_total() const { return (int64_t)cvals_host_size_; }
 
@@ -56,14 +67,16 @@ class BatchLocalization {
   double eps_ = 0;
   size_t cvals_host_size_ = 0;
   hipStream_t stream_ = nullptr;
-  hipGraphExec_t graph_ = nullptr;
+  hipGraphExec_t graphs_[kConfigs] = {};
+  int config_ = 0;
+  DBuf<int32_t> order_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   DBuf<int32_t> pose_begin_, pose_soff_, match_ptr_, match_args_, odo_args_;
   DBuf<double> cvals_, vout_, vals_, lin_, maxd_, err_;
   DBuf<symhip_iteration_stats> recs_;
   DBuf<symhip_optimization_stats> stats_;
   DBuf<int64_t> ticks_;
-  void build_graph();
+  void build_graph(int c);
 };
 
 }  // namespace symhip

@@ -6,8 +6,8 @@
 // Structure exploited: the optimized keys are the poses of one chain, odometry factors join
 // consecutive poses and matching factors touch one pose, so H is block tridiagonal with 6 x 6
 // blocks. Lane i linearizes pose i (its matching factors and the odometry factor to pose i + 1,
-// whose pose-(i + 1) terms are handed to lane i + 1 by a shuffle); lane 0 then solves the damped
-// system by block-tridiagonal Cholesky from LDS. Per state block and pose the linearization keeps
+// whose pose-(i + 1) terms are handed to lane i + 1 by a shuffle); the wavefront then solves the
+// damped system by block-tridiagonal Cholesky, six lanes holding the rows of each 6 x 6 block. Per state block and pose the linearization keeps
 // 63 doubles: the lower diagonal block (21), H(i + 1, i) (36, row-major) and the rhs (6).
 #include "./batch.h"
 
@@ -45,6 +45,7 @@ struct BatchView {
   int32_t rec_stride;
   symhip_optimization_stats* stats;
   int64_t* ticks;  // [n_problems][2]
+  const int32_t* order;  // workgroup -> problem (null: the identity)
 };
 
 __device__ __forceinline__ double wave_total(double v) {
@@ -168,76 +169,141 @@ __device__ double linearize(const BatchView& v, int b, int64_t g, int lane, bool
   return 0.5 * wave_total(e);
 }
 
-// Block-tridiagonal Cholesky H = L L^T (L block lower bidiagonal: C_i on the diagonal,
-// W_i = B_i C_i^-T below) and the two triangular solves. A: packed lower diagonal blocks
-// (overwritten by C_i), B: H(i + 1, i) row-major (overwritten by W_i), x: rhs in, H^-1 rhs out.
-__device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36], double (*x)[6]) {
-  for (int i = 0; i < n; ++i) {
-    double M[6][6];
-    for (int r = 0; r < 6; ++r)
-      for (int c = 0; c <= r; ++c) M[r][c] = A[i][tri(r, c)];
-    if (i > 0) {
-      const double* W = B[i - 1];
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c <= r; ++c) {
-          double s = 0;
-          for (int k = 0; k < 6; ++k) s += W[r * 6 + k] * W[c * 6 + k];
-          M[r][c] -= s;
-        }
-    }
-    for (int k = 0; k < 6; ++k) {
-      M[k][k] = sqrt(M[k][k]);
-      for (int r = k + 1; r < 6; ++r) M[r][k] /= M[k][k];
-      for (int r = k + 1; r < 6; ++r)
-        for (int c = k + 1; c <= r; ++c) M[r][c] -= M[r][k] * M[c][k];
-    }
-    for (int r = 0; r < 6; ++r)
-      for (int c = 0; c <= r; ++c) A[i][tri(r, c)] = M[r][c];
-    if (i + 1 < n) {
-      double* W = B[i];
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) {
-          double w = W[r * 6 + c];
-          for (int k = 0; k < c; ++k) w -= W[r * 6 + k] * M[c][k];
-          W[r * 6 + c] = w / M[c][c];
-        }
-    }
-  }
-  for (int i = 0; i < n; ++i) {
-    double b[6];
-    for (int r = 0; r < 6; ++r) {
-      b[r] = x[i][r];
-      if (i > 0)
-        for (int k = 0; k < 6; ++k) b[r] -= B[i - 1][r * 6 + k] * x[i - 1][k];
-    }
-    for (int r = 0; r < 6; ++r) {
-      double s = b[r];
-      for (int k = 0; k < r; ++k) s -= A[i][tri(r, k)] * x[i][k];
-      x[i][r] = s / A[i][tri(r, r)];
-    }
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    double b[6];
-    for (int r = 0; r < 6; ++r) {
-      b[r] = x[i][r];
-      if (i + 1 < n)
-        for (int k = 0; k < 6; ++k) b[r] -= B[i][k * 6 + r] * x[i + 1][k];
-    }
-    for (int r = 5; r >= 0; --r) {
-      double s = b[r];
-      for (int k = r + 1; k < 6; ++k) s -= A[i][tri(k, r)] * x[i][k];
-      x[i][r] = s / A[i][tri(r, r)];
-    }
-  }
+// Value of v in lane k (k wave-uniform) through two scalar lane reads.
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+  const int2 u = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(u.x, k),
+                                              __builtin_amdgcn_readlane(u.y, k)));
 }
 
-template <typename S>
-__global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimizer_params p,
-                                                      double eps) {
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block-tridiagonal Cholesky H = L L^T (L block lower bidiagonal: C_i on the diagonal,
+// W_i = B_i C_i^-T below) and the two triangular solves, by the whole wavefront: lane r < 6
+// holds row r of the 6 x 6 blocks of the current pose (the other lanes shadow row 5), and the
+// values another row needs cross by scalar lane reads, so a pose costs a few dozen dependent
+// steps instead of the ~6 000 cycles of one lane doing every entry. Each entry is formed by the
+// same operations in the same order as a one-lane sweep would form it. A: packed lower diagonal
+// blocks (overwritten by C_i), B: H(i + 1, i) row-major (overwritten by W_i), x: rhs in,
+// H^-1 rhs out (LDS, written by the row lanes, read back after a wavefront fence).
+__device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36], double (*x)[6],
+                                        int lane) {
+  const int r = lane < 6 ? lane : 5;
+  const bool row = lane < 6;
+  double wprev[6] = {}, yprev[6] = {};
+  for (int i = 0; i < n; ++i) {
+    // M = A_i - W_{i-1} W_{i-1}^T, row r (entries c <= r)
+    double m[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) m[c] = A[i][tri(r, c <= r ? c : r)];
+    if (i > 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double sum = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sum += wprev[k] * B[i - 1][c * 6 + k];
+        m[c] -= sum;
+      }
+    }
+    // Cholesky of M; every entry of C_i ends up in Cb (uniform) on the way
+    double Cb[6][6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double ckk = sqrt(lane_bcast(m[k], k));
+      Cb[k][k] = ckk;
+      const double lrk = m[k] / ckk;  // row r's entry of column k (r > k)
+#pragma unroll
+      for (int c = k + 1; c < 6; ++c) Cb[c][k] = lane_bcast(lrk, c);
+      if (r > k) m[k] = lrk;
+      else if (r == k) m[k] = ckk;
+#pragma unroll
+      for (int c = k + 1; c < 6; ++c)
+        if (c <= r && r > k) m[c] -= lrk * Cb[c][k];
+    }
+    if (row) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        if (c <= r) A[i][tri(r, c)] = m[c];
+    }
+    // W_i row r = B_i row r C_i^-T, and y_i = C_i^-1 (x_i - W_{i-1} y_{i-1})
+    double w[6];
+    if (i + 1 < n) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double v = B[i][r * 6 + c];
+#pragma unroll
+        for (int k = 0; k < c; ++k) v -= w[k] * Cb[c][k];
+        w[c] = v / Cb[c][c];
+      }
+    }
+    double b = x[i][r];
+    if (i > 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) b -= wprev[k] * yprev[k];
+    }
+    double bu[6], y[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bu[k] = lane_bcast(b, k);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double v = bu[q];
+#pragma unroll
+      for (int k = 0; k < q; ++k) v -= Cb[q][k] * y[k];
+      y[q] = v / Cb[q][q];
+    }
+    if (row) {
+      x[i][r] = y[r < 6 ? r : 5];
+      if (i + 1 < n) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) B[i][r * 6 + c] = w[c];
+      }
+    }
+    wave_lds_sync();  // W_i is read by all rows at pose i + 1
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      wprev[k] = w[k];
+      yprev[k] = y[k];
+    }
+  }
+  // x_i = C_i^-T (y_i - W_i^T x_{i+1})
+  double xnext[6] = {};
+  for (int i = n - 1; i >= 0; --i) {
+    double b = x[i][r];
+    if (i + 1 < n) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) b -= B[i][k * 6 + r] * xnext[k];
+    }
+    double bu[6], xs[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bu[k] = lane_bcast(b, k);
+#pragma unroll
+    for (int q = 5; q >= 0; --q) {
+      double v = bu[q];
+#pragma unroll
+      for (int k = q + 1; k < 6; ++k) v -= A[i][tri(k, q)] * xs[k];
+      xs[q] = v / A[i][tri(q, q)];
+    }
+    if (row) x[i][r] = xs[r < 6 ? r : 5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xnext[k] = xs[k];
+  }
+  wave_lds_sync();
+}
+
+// kWaves: wavefronts per SIMD the register allocation must allow (1: no spills, the linearization
+// holds ~350 VGPRs; 2: twice the resident problems at the cost of ~120 spilled registers); the
+// batch autotuner picks per batch.
+template <typename S, int kWaves>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves)))
+batch_lm_kernel(BatchView v, symhip_optimizer_params p, double eps) {
   __shared__ double sA[kMaxPoses][21];
   __shared__ double sB[kMaxPoses][36];
   __shared__ double sx[kMaxPoses][6];
-  const int prob = blockIdx.x, lane = threadIdx.x;
+  const int prob = v.order ? v.order[blockIdx.x] : (int)blockIdx.x, lane = threadIdx.x;
   const int64_t t_start = wall_clock64();
   const int g0 = v.pose_begin[prob], n = v.pose_begin[prob + 1] - g0;
   const bool live = lane < n;
@@ -312,7 +378,7 @@ __global__ void __launch_bounds__(64) batch_lm_kernel(BatchView v, symhip_optimi
     }
     if (p.use_diagonal_damping && p.keep_max_diagonal_damping) have_max = true;
     __syncthreads();
-    if (lane == 0) block_tridiagonal_solve(n, sA, sB, sx);
+    block_tridiagonal_solve(n, sA, sB, sx, lane);
     __syncthreads();
     double lde = 0;
     if (live) {
@@ -453,40 +519,81 @@ BatchLocalization::BatchLocalization(const symhip_problem_desc* descs, int32_t n
   recs_.resize((size_t)n_problems_ * (p_.iterations + 1));
   stats_.resize(n_problems_);
   ticks_.resize(2 * (size_t)n_problems_);
+  {
+    std::vector<int32_t> order(n_problems_);
+    for (int32_t i = 0; i < n_problems_; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return pose_begin[a + 1] - pose_begin[a] > pose_begin[b + 1] - pose_begin[b];
+    });
+    order_.upload(order, stream_);
+  }
   HIP_OK(hipStreamSynchronize(stream_));
-  build_graph();
+  build_graph(config_);
 }
 
 BatchLocalization::~BatchLocalization() {
-  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (auto& g : graphs_)
+    if (g) (void)hipGraphExecDestroy(g);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void BatchLocalization::build_graph() {
+// Launch configuration c: bit 0 = two wavefronts per SIMD (else one), bit 1 = problems with
+// the most poses first (else in the given order).
+void BatchLocalization::build_graph(int c) {
+  if (graphs_[c]) return;
   BatchView v{total_poses_,      pose_begin_.get(), pose_soff_.get(), match_ptr_.get(),
               match_args_.get(), odo_args_.get(),   cvals_.get(),     vals_.get(),
               lin_.get(),        maxd_.get(),       vout_.get(),      recs_.get(),
-              p_.iterations + 1, stats_.get(),      ticks_.get()};
+              p_.iterations + 1, stats_.get(),      ticks_.get(),
+              (c & 2) ? order_.get() : nullptr};
   hipGraph_t graph;
   HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  if (f32_)
-    hipLaunchKernelGGL(batch_lm_kernel<float>, dim3(n_problems_), dim3(64), 0, stream_, v, p_,
-                       eps_);
-  else
-    hipLaunchKernelGGL(batch_lm_kernel<double>, dim3(n_problems_), dim3(64), 0, stream_, v, p_,
-                       eps_);
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(n_problems_), dim3(64), 0, stream_, v, p_, eps_);
+  };
+  if (f32_) {
+    if (c & 1) launch(batch_lm_kernel<float, 2>); else launch(batch_lm_kernel<float, 1>);
+  } else {
+    if (c & 1) launch(batch_lm_kernel<double, 2>); else launch(batch_lm_kernel<double, 1>);
+  }
   HIP_OK(hipStreamEndCapture(stream_, &graph));
-  HIP_OK(hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0));
+  HIP_OK(hipGraphInstantiate(&graphs_[c], graph, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(graph));
+}
+
+// The autotuner: every configuration solves the batch `reps` times (the solves are complete and
+// give the same results: the configurations change occupancy and order, not the arithmetic) and
+// the one with the shortest launch is kept for later solves.
+void BatchLocalization::tune(int reps, double* config_ms) {
+  SYMHIP_CHECK(reps >= 1, "batch: tune needs reps >= 1");
+  double best = 0;
+  for (int c = 0; c < kConfigs; ++c) {
+    build_graph(c);
+    double ms = 0;
+    for (int r = 0; r < reps; ++r) {
+      HIP_OK(hipEventRecord(ev0_, stream_));
+      HIP_OK(hipGraphLaunch(graphs_[c], stream_));
+      HIP_OK(hipEventRecord(ev1_, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      float t = 0;
+      HIP_OK(hipEventElapsedTime(&t, ev0_, ev1_));
+      ms = r == 0 ? t : std::min<double>(ms, t);
+    }
+    if (config_ms) config_ms[c] = ms;
+    if (c == 0 || ms < best) {
+      best = ms;
+      config_ = c;
+    }
+  }
 }
 
 void BatchLocalization::solve(double* values_out, symhip_optimization_stats* stats_out,
                               symhip_iteration_stats* iters_out, double* solve_ms,
                               double* total_ms) {
   HIP_OK(hipEventRecord(ev0_, stream_));
-  HIP_OK(hipGraphLaunch(graph_, stream_));
+  HIP_OK(hipGraphLaunch(graphs_[config_], stream_));
   HIP_OK(hipEventRecord(ev1_, stream_));
   if (values_out) vout_.download(values_out, cvals_host_size_, stream_);
   if (stats_out) stats_.download(stats_out, n_problems_, stream_);
@@ -581,13 +688,13 @@ void batch_solve_noalloc(const symhip_batch_tables& t, const symhip_optimizer_pa
                      dim3(256), 0, stream, t.storage_size, t.values, values_out);
   BatchView v{TP,          t.pose_begin, t.pose_soff, t.match_ptr,        t.match_args,
               t.odo_args,  t.values,     vals,        lin,                maxd,
-              values_out,  iters_out,    p.iterations + 1, stats_out,     ticks};
+              values_out,  iters_out,    p.iterations + 1, stats_out,     ticks, nullptr};
   if (precision == SYMHIP_PRECISION_F32_JAC)
-    hipLaunchKernelGGL(batch_lm_kernel<float>, dim3(t.n_problems), dim3(64), 0, stream, v, p,
-                       epsilon);
+    hipLaunchKernelGGL((batch_lm_kernel<float, 1>), dim3(t.n_problems), dim3(64), 0, stream, v,
+                       p, epsilon);
   else
-    hipLaunchKernelGGL(batch_lm_kernel<double>, dim3(t.n_problems), dim3(64), 0, stream, v, p,
-                       epsilon);
+    hipLaunchKernelGGL((batch_lm_kernel<double, 1>), dim3(t.n_problems), dim3(64), 0, stream, v,
+                       p, epsilon);
   HIP_OK(hipGetLastError());
 }
 

@@ -610,6 +610,21 @@ int symhip_batch_solve_noalloc(const symhip_batch_tables* tables,
   });
 }
 
+int symhip_batch_tune(symhip_batch_t batch, int32_t reps, double* config_ms, int32_t* chosen) {
+  return guarded([&] {
+    SYMHIP_CHECK(batch, "invalid arguments");
+    batch->b->tune(reps, config_ms);
+    if (chosen) *chosen = batch->b->config();
+  });
+}
+
+int symhip_batch_set_config(symhip_batch_t batch, int32_t config) {
+  return guarded([&] {
+    SYMHIP_CHECK(batch, "invalid arguments");
+    batch->b->set_config(config);
+  });
+}
+
 int symhip_batch_solve(symhip_batch_t batch, double* values_out,
                        symhip_optimization_stats* stats_out, symhip_iteration_stats* iters_out,
                        double* solve_ms, double* total_ms) {

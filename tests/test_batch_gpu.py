@@ -91,3 +91,26 @@ def test_full_batch_of_4096_problems_matches_oracle(lam):
         assert_same_trajectory(got, ora, rtol=1e-8)
         np.testing.assert_allclose(got["values"], ora["values"], rtol=0,
                                    atol=1e-8 * max(1.0, np.abs(ora["values"]).max()))
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_F64, capi.PRECISION_F32_JAC])
+def test_every_launch_configuration_gives_the_same_results(precision):
+    """The autotuner's configurations (occupancy one or two wavefronts per SIMD, problems as
+    given or most poses first) change where and when problems run, not their arithmetic: every
+    configuration's solve is bit-identical, and tuning keeps one of them."""
+    probs = capi.gen_localization_batch(300, seed=8)
+    b = capi.HipBatch(probs, capi.default_params(iterations=50), precision=precision)
+    ref = b.solve()
+    for c in range(4):
+        b.set_config(c)
+        r = b.solve()
+        for a, x in zip(ref["problems"], r["problems"]):
+            np.testing.assert_array_equal(a["values"], x["values"])
+            assert (a["status"], a["best_index"]) == (x["status"], x["best_index"])
+            assert [it["new_error"] for it in a["iterations"]] == \
+                [it["new_error"] for it in x["iterations"]]
+    chosen, ms = b.tune(reps=1)
+    assert 0 <= chosen < 4 and np.all(ms > 0) and ms[chosen] == ms.min()
+    r = b.solve()
+    for a, x in zip(ref["problems"], r["problems"]):
+        np.testing.assert_array_equal(a["values"], x["values"])
