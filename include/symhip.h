@@ -153,6 +153,7 @@ enum symhip_ordering {
 enum symhip_precision {
   SYMHIP_PRECISION_F64 = 0,      /* plain FP64 everywhere */
   SYMHIP_PRECISION_F32_JAC = 1,  /* FP32 Jacobians/Hessian blocks, FP64 residual/cost/decisions */
+  SYMHIP_PRECISION_F32_UPDATES = 2, /* sparse Cholesky path: mixed-precision supernodal LDL^T */
 };
 
 typedef struct symhip_solver_options {
@@ -163,7 +164,15 @@ typedef struct symhip_solver_options {
                             from FP32 factor bodies, W and Z = W L_p^-T are stored in FP32 and so
                             is S for PCG; residual, gradient, matrix-core sums, LDL^T, cost and
                             gain ratio stay FP64. Not combinable with the learned preconditioner
-                            or system logging (both need FP64 systems). */
+                            or system logging (both need FP64 systems).
+                            SYMHIP_PRECISION_F32_UPDATES: sparse Cholesky path only; the
+                            trailing supernode updates of the LDL^T run on the FP32 matrix cores
+                            (diagonal blocks, panel solves, D and the triangular solves FP64) and
+                            every solve is refined against the FP64 damped Hessian by
+                            conjugate gradients preconditioned by that factor (FP64 products,
+                            residuals and scalars) to a relative residual of 1e-13 or its
+                            stagnation below 1e-10, at most 30 iterations; a solve that misses
+                            it refactors with FP64 updates. */
   int32_t use_pcg;       /* Schur path: block-Jacobi preconditioned CG on the reduced camera
                             system. 0 off (direct LDL^T), 1 truncated (an iterate that missed
                             pcg_rel_tol within pcg_max_iters is used; LDL^T only when CG
@@ -255,8 +264,17 @@ int symhip_optimizer_phase_times(symhip_optimizer_t opt, double* total_ms, int64
 int symhip_optimizer_solver_info(symhip_optimizer_t opt, int64_t* out10);
 /* Iterative (PCG) linear solver statistics since creation: out[0]=solves, out[1]=total CG
    iterations, out[2]=fallbacks to the direct LDL^T, out[3]=most iterations in one solve.
-   No reference counterpart (the reference solvers are direct). */
+   No reference counterpart (the reference solvers are direct). With
+   SYMHIP_PRECISION_F32_UPDATES: out[0]=solves, out[1]=refinement (CG) iterations,
+   out[2]=fallbacks to the FP64 factorization, out[3]=most iterations in one solve. */
 int symhip_optimizer_linear_solver_stats(symhip_optimizer_t opt, int64_t* out4);
+/* Mixed-precision sparse Cholesky (SYMHIP_PRECISION_F32_UPDATES) only; no reference counterpart.
+   track >= 0 turns the recording of every solve's final relative residual |b - A x| / |b|
+   (A the damped Hessian, FP64) on (1) or off (0) and clears the record; track < 0 only reads.
+   Copies up to max recorded residuals into out (may be NULL); returns the number recorded, or
+   -1 on error. */
+int32_t symhip_optimizer_refinement_residuals(symhip_optimizer_t opt, int32_t track, double* out,
+                                              int32_t max);
 
 /* Learned PCG preconditioner (Schur path, use_pcg != 0, 9-dof cameras). No reference
    counterpart. The network has SYMHIP_LEARNED_PRECOND_PARAMS float parameters. */
@@ -365,6 +383,11 @@ int symhip_cholesky_create(int32_t n, const int32_t* colptr, const int32_t* rowi
 int symhip_cholesky_destroy(symhip_cholesky_t c);
 /* Factorize(A) with host values in the CSC order given at creation. */
 int symhip_cholesky_factorize(symhip_cholesky_t c, const double* values);
+/* Mixed-precision factorization (no reference counterpart): on != 0 runs the trailing supernode
+   updates on the FP32 matrix cores (diagonal blocks, panels, D and the solves stay FP64). The
+   factor is then FP32-accurate in the updates; Solve does not refine (the optimizer's
+   SYMHIP_PRECISION_F32_UPDATES form does). Call before the first Factorize. */
+int symhip_cholesky_set_f32_updates(symhip_cholesky_t c, int32_t on);
 /* Solve(b) for nrhs column-major right-hand sides (host buffers). */
 int symhip_cholesky_solve(symhip_cholesky_t c, const double* b, double* x, int32_t nrhs);
 /* inv_permutation (new -> old), the reference's Permutation().inverse() convention. */

@@ -410,7 +410,7 @@ PHASE_NAMES = ["linearize", "damp", "point_elim", "rcs_assemble", "reduced_rhs",
                "solve", "back_sub", "retract", "decide", "lin_points", "lin_cams"]
 NUM_PHASES = len(PHASE_NAMES)
 ORDERING_NATURAL, ORDERING_MIN_DEGREE, ORDERING_NESTED_DISSECTION = 0, 1, 2
-PRECISION_F64, PRECISION_F32_JAC = 0, 1
+PRECISION_F64, PRECISION_F32_JAC, PRECISION_F32_UPDATES = 0, 1, 2
 PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
 LEARNED_PRECOND_PARAMS = 21744
 
@@ -715,6 +715,16 @@ class HipOptimizer:
             ptr(err, C.c_double)))
         return int(it[0]), float(err[0])
 
+    def refinement_residuals(self, track=-1, max_count=4096):
+        """Mixed-precision sparse Cholesky: track 1/0 turns the per-solve record of the final
+        relative residual |b - A x| / |b| on/off (and clears it); returns the recorded values."""
+        out = np.zeros(max_count)
+        n = hip_lib().symhip_optimizer_refinement_residuals(self.h, track, ptr(out, C.c_double),
+                                                            max_count)
+        if n < 0:
+            check(-1)
+        return out[:min(n, max_count)]
+
     def linear_solver_stats(self):
         out = np.zeros(4, np.int64)
         check(hip_lib().symhip_optimizer_linear_solver_stats(self.h, ptr(out, C.c_int64)))
@@ -817,6 +827,10 @@ class HipCholesky:
         if getattr(self, "h", None):
             hip_lib().symhip_cholesky_destroy(self.h)
             self.h = None
+
+    def set_f32_updates(self, on=True):
+        """Mixed-precision factorization (trailing updates on the FP32 matrix cores)."""
+        check(hip_lib().symhip_cholesky_set_f32_updates(self.h, int(on)))
 
     def inv_permutation(self):
         out = np.zeros(self.n, np.int32)

@@ -376,6 +376,13 @@ int symhip_optimizer_linear_solver_stats(symhip_optimizer_t o, int64_t* out) {
   return guarded([&] { o->opt->backend().linear_stats(out); });
 }
 
+int32_t symhip_optimizer_refinement_residuals(symhip_optimizer_t o, int32_t track, double* out,
+                                              int32_t max) {
+  int32_t n = -1;
+  guarded([&] { n = o->opt->backend().refinement_residuals(track, out, max, o->opt->stream()); });
+  return n;
+}
+
 int symhip_optimizer_log_systems(symhip_optimizer_t o, int32_t max) {
   return guarded([&] { o->opt->backend().log_systems(max); });
 }
@@ -466,6 +473,13 @@ int symhip_cholesky_factorize(symhip_cholesky_t c, const double* values) {
     c->vals.upload(values, c->rowind.size(), c->stream);
     c->ldlt.factorize(c->vals.get(), c->stream);
     HIP_OK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int symhip_cholesky_set_f32_updates(symhip_cholesky_t c, int32_t on) {
+  return guarded([&] {
+    ensure_cholesky_device(c);
+    c->ldlt.set_f32_updates(on != 0);
   });
 }
 

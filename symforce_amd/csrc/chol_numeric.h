@@ -33,6 +33,12 @@ class SupernodalLDLT {
   SupernodalLDLT& operator=(const SupernodalLDLT&) = delete;
   ~SupernodalLDLT();
 
+  // Mixed-precision form: the trailing supernode updates (update_kernel) run on the FP32 matrix
+  // cores; diagonal blocks, panel solves, D and the triangular solves stay FP64. The factor is
+  // then accurate to FP32 in the updates, and callers refine solves against the FP64 matrix.
+  // May change between factorizations (each form's launch sequence is captured on first use).
+  void set_f32_updates(bool on) { f32_updates_ = on; }
+  bool f32_updates() const { return f32_updates_; }
   double* panels() const { return panel_.get(); }
   const CholSymbolic& symbolic() const { return sym_; }
   // Downloads the panels (L21 below the diagonal blocks; the diagonal blocks keep A11) and D.
@@ -50,6 +56,7 @@ class SupernodalLDLT {
   struct CachedGraph {
     const double* b;
     double* x;
+    bool f32;  // factorization graphs: the update form captured
     hipGraphExec_t exec;
   };
   template <typename F>
@@ -58,6 +65,7 @@ class SupernodalLDLT {
   // captured launch sequences, one per argument pair seen (a few state blocks)
   std::vector<CachedGraph> fact_graphs_, solve_graphs_, bwd_graphs_;
   bool fused_ = false;  // the last factorization ran the forward solve
+  bool f32_updates_ = false;
   CholSymbolic sym_;
   DBuf<double> panel_, l11_, D_, y_, yout_, part_;
   DBuf<int32_t> counter_;  // per-supernode arrival counters of the backward pass (self-resetting)

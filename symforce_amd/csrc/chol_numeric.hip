@@ -1,6 +1,7 @@
 // Device numeric supernodal LDL^T: assembly, per-level fused diagonal-block factorization + panel
 // triangular solve, right-looking update (SYRK-like) scattered into ancestor panels, and the
-// triangular solves. All FP64.
+// triangular solves. FP64, except the trailing updates of the mixed-precision form (FP32 matrix
+// cores, set_f32_updates).
 //
 // The work per supernode is small (panels are at most 64 columns wide), so every kernel is
 // latency-bound: the kernels are written to avoid data-dependent branches (the 64 elimination
@@ -290,15 +291,22 @@ __global__ void __launch_bounds__(kDiagThreads) diag_trsm_kernel(
 // 32-row tiles: a block's product (K = 64 on the FP64 matrix cores, 4096 cycles for a 64-row
 // tile on one CU) is the latency of such a level, and a quarter of it spreads over four times
 // the blocks.
-template <int KT>
+//
+// T = float is the mixed-precision form of the update: both row tiles are rounded to FP32 (the
+// D scaling applied first, in FP64), the product runs on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32, FP32 accumulation) and is scattered as FP64 like the FP64 form; the
+// diagonal blocks and panels stay FP64 (diag_trsm_kernel), and the caller refines its solves
+// against the FP64 matrix.
+template <int KT, typename T>
 __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict__ panel,
                                             const double* __restrict__ D,
                                             const int64_t* __restrict__ scat_col,
                                             const int32_t* __restrict__ scat_rel,
                                             const int32_t* __restrict__ upd_rel, double* smem) {
   static_assert(KT == 64 || KT == 32, "update tiles are 64 or 32 rows");
-  auto As = reinterpret_cast<double (*)[KT + 1]>(smem);
-  auto Bs = reinterpret_cast<double (*)[KT + 1]>(smem + kMaxCols * (KT + 1));
+  constexpr bool kF64 = sizeof(T) == 8;
+  auto As = reinterpret_cast<T (*)[KT + 1]>(smem);
+  auto Bs = reinterpret_cast<T (*)[KT + 1]>(reinterpret_cast<T*>(smem) + kMaxCols * (KT + 1));
   const SnDesc sd = tk.d;
   const int ti = tk.a, tj = tk.b;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
@@ -314,15 +322,16 @@ __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict
     const int kc = min(k, n - 1);
     const double va = P[(int64_t)kc * ld + n + min(ri, mb - 1)];
     const double vb = P[(int64_t)kc * ld + n + min(rj, mb - 1)] * Ds[kc];
-    As[k][r] = (k < n && ri < mb) ? va : 0.0;
-    Bs[k][r] = (k < n && rj < mb) ? vb : 0.0;
+    As[k][r] = (k < n && ri < mb) ? T(va) : T(0);
+    Bs[k][r] = (k < n && rj < mb) ? T(vb) : T(0);
   }
   // The product on the FP64 matrix cores, in 16 x 16 output blocks, K = 64 in 16 steps of
   // v_mfma_f64_16x16x4f64: a 64-row tile gives wave w row block w and all four column blocks, a
   // 32-row tile gives it row block w / 2 and column block w % 2. Lane l feeds
   // A(i = l % 16, k = l / 16) = As[k][16 rb + i] and B(k, j = l % 16) = Bs[k][16 c + j] and holds
-  // D(l / 16 + 4 r, l % 16) of block c in accumulator register r.
-  typedef double d4 __attribute__((ext_vector_type(4)));
+  // D(l / 16 + 4 r, l % 16) of block c in accumulator register r (FP64) or D(4 (l / 16) + r,
+  // l % 16) (FP32).
+  typedef T d4 __attribute__((ext_vector_type(4)));
   constexpr int NBW = KT == 64 ? 4 : 1;  // output blocks per wave
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar metadata
@@ -353,32 +362,39 @@ __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict
   d4 acc[NBW] = {};
 #pragma unroll 4
   for (int k0 = 0; k0 < kMaxCols; k0 += 4) {
-    const double av = As[k0 + lk][16 * rb + lr];
-    double bv[NBW];
+    const T av = As[k0 + lk][16 * rb + lr];
+    T bv[NBW];
 #pragma unroll
     for (int c = 0; c < NBW; ++c) bv[c] = Bs[k0 + lk][16 * (c0 + c) + lr];
 #pragma unroll
-    for (int c = 0; c < NBW; ++c)
-      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
+    for (int c = 0; c < NBW; ++c) {
+      if constexpr (kF64)
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
+      else
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], acc[c], 0, 0, 0);
+    }
   }
-  // transpose through LDS (the A tile's space): T[column][row]
+  // transpose through LDS (the tiles' space, as FP64): Tr[column][row]
   __syncthreads();
-  auto T = As;
+  auto Tr = reinterpret_cast<double (*)[KT + 1]>(smem);
 #pragma unroll
   for (int c = 0; c < NBW; ++c)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) T[16 * (c0 + c) + lr][16 * rb + lk + 4 * r] = acc[c][r];
+    for (int r = 0; r < 4; ++r) {
+      const int row = kF64 ? lk + 4 * r : 4 * lk + r;
+      Tr[16 * (c0 + c) + lr][16 * rb + row] = double(acc[c][r]);
+    }
   __syncthreads();
   const int i = ti * KT + srow;
 #pragma unroll
   for (int q = 0; q < kNq; ++q) {
     const int jl = wave + 4 * (kCpi * q + ssub), j = tj * KT + jl;
-    const double v = T[jl][srow];
+    const double v = Tr[jl][srow];
     if (j < mb && i < mb && i >= j) unsafeAtomicAdd(panel + cbase[q] + rel[q], -v);
   }
 }
 
-template <int KT>
+template <int KT, typename T>
 __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ tasks,
                                                      double* __restrict__ panel,
                                                      const double* __restrict__ D,
@@ -386,7 +402,7 @@ __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ 
                                                      const int32_t* __restrict__ scat_rel,
                                                      const int32_t* __restrict__ upd_rel) {
   __shared__ double smem[kUpdSmem];
-  update_body<KT>(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
+  update_body<KT, T>(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
 // y = P b; also clears the backward pass's partial sums.
@@ -724,12 +740,12 @@ template <typename F>
 hipGraphExec_t SupernodalLDLT::cached_graph(std::vector<CachedGraph>& cache, const double* b,
                                             double* x, hipStream_t s, F&& launches) {
   for (const auto& g : cache)
-    if (g.b == b && g.x == x) return g.exec;
+    if (g.b == b && g.x == x && g.f32 == f32_updates_) return g.exec;
   if (cache.size() >= 8) {  // bound the cache; the oldest goes
     (void)hipGraphExecDestroy(cache.front().exec);
     cache.erase(cache.begin());
   }
-  cache.push_back({b, x, capture(s, launches)});
+  cache.push_back({b, x, f32_updates_, capture(s, launches)});
   return cache.back().exec;
 }
 
@@ -753,14 +769,11 @@ void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
     }
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
     if (nu) {
-      if (lvl_upd32_[l])
-        hipLaunchKernelGGL(update_kernel<kTile / 2>, dim3(nu), dim3(256), 0, s,
-                           upd + lvl_upd_ptr_[l], panel_.get(), D_.get(), scat_col_.get(),
-                           scat_rel_.get(), upd_rel_.get());
-      else
-        hipLaunchKernelGGL(update_kernel<kTile>, dim3(nu), dim3(256), 0, s,
-                           upd + lvl_upd_ptr_[l], panel_.get(), D_.get(), scat_col_.get(),
-                           scat_rel_.get(), upd_rel_.get());
+      const bool t32 = lvl_upd32_[l];
+      auto kern = f32_updates_ ? (t32 ? update_kernel<kTile / 2, float> : update_kernel<kTile, float>)
+                               : (t32 ? update_kernel<kTile / 2, double> : update_kernel<kTile, double>);
+      hipLaunchKernelGGL(kern, dim3(nu), dim3(256), 0, s, upd + lvl_upd_ptr_[l], panel_.get(),
+                         D_.get(), scat_col_.get(), scat_rel_.get(), upd_rel_.get());
     }
   }
   HIP_OK(hipGetLastError());

@@ -66,6 +66,12 @@ class SystemBackend {
                                std::vector<int32_t>& l_rowind) const {
     throw Error("symhip: the factor sparsity is available on the sparse Cholesky path only");
   }
+  // Mixed-precision sparse Cholesky only: track >= 0 turns the recording of each solve's final
+  // relative residual on (1) or off (0) and clears the record; returns the number recorded and
+  // copies up to max of them into out (when non-null).
+  virtual int refinement_residuals(int track, double* out, int max, hipStream_t s) {
+    throw Error("symhip: refinement residuals need SYMHIP_PRECISION_F32_UPDATES");
+  }
   // Called at the start of every optimize run and after every LM iteration record.
   virtual void begin_run() {}
   virtual void observe(const symhip_iteration_stats& rec) { (void)rec; }

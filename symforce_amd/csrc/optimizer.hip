@@ -322,6 +322,45 @@ __global__ void negate_kernel(int32_t n, const double* __restrict__ x, double* _
   if (i < n) y[i] = -x[i];
 }
 
+// out = b - A x (b non-null) or out = A x, for the symmetric A held as its lower CSC (colptr,
+// rowind, vals): row i of A is column i's entries (rows >= i) and the entries (i, j), j < i,
+// which tptr / tcol / tidx list per row (column j and the entry's index in vals). One thread
+// per row, FP64, no atomics: the residuals and products of the mixed-precision form's
+// refinement.
+__global__ void __launch_bounds__(256) sym_matvec_kernel(
+    int32_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+    const double* __restrict__ vals, const int32_t* __restrict__ tptr,
+    const int32_t* __restrict__ tcol, const int64_t* __restrict__ tidx,
+    const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int32_t e = colptr[i]; e < colptr[i + 1]; ++e) acc += vals[e] * x[rowind[e]];
+  for (int32_t e = tptr[i]; e < tptr[i + 1]; ++e) acc += vals[tidx[e]] * x[tcol[e]];
+  out[i] = b ? b[i] - acc : acc;
+}
+
+// CG step with the scalars on the device: alpha = rz / pq; x += alpha p; r -= alpha q.
+__global__ void cg_step_kernel(int32_t n, const double* __restrict__ rz,
+                               const double* __restrict__ pq, const double* __restrict__ p,
+                               const double* __restrict__ q, double* __restrict__ x,
+                               double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = *rz / *pq;
+  x[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+
+// p = z + beta p with beta = rz_new / rz_old (p = z when rz_old is null).
+__global__ void cg_direction_kernel(int32_t n, const double* __restrict__ rz_new,
+                                    const double* __restrict__ rz_old,
+                                    const double* __restrict__ z, double* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = rz_old ? z[i] + (*rz_new / *rz_old) * p[i] : z[i];
+}
+
 }  // namespace
 
 // -------------------------------------------------------------------------------------------
@@ -355,6 +394,10 @@ class CscBackend final : public SystemBackend {
     N_ = p.N;
     M_ = p.M;
     nnz_ = p.nnz();
+    if (o.precision == SYMHIP_PRECISION_F32_UPDATES) setup_refinement(p, s);
+  }
+  ~CscBackend() override {
+    if (host_sums_) (void)hipHostFree(host_sums_);
   }
   void relinearize(int b, const double* values, double* err_out, hipStream_t s) override {
     lin_.run(values, residual_[b].get(), hval_[b].get(), rhs_[b].get(), s);
@@ -380,8 +423,29 @@ class CscBackend final : public SystemBackend {
     }
     PhaseScope ps(timer_, SYMHIP_PHASE_SOLVE, s);
     chol_.solve_fused(x_.get(), s);
+    if (chol_.f32_updates()) refine(b, s);
     hipLaunchKernelGGL(negate_kernel, dim3(grid_for(N_, 256)), dim3(256), 0, s, N_, x_.get(),
                        update);
+  }
+  // Mixed-precision form: out[0] solves, out[1] refinement (CG) iterations, out[2] fallbacks to
+  // the FP64 factorization, out[3] most iterations in one solve.
+  void linear_stats(int64_t* out4) const override {
+    out4[0] = solves_;
+    out4[1] = refine_steps_;
+    out4[2] = fallbacks_;
+    out4[3] = max_refine_steps_;
+  }
+  // With tracking on, every mixed-precision solve also records its final relative residual.
+  int refinement_residuals(int track, double* out, int max, hipStream_t s) override {
+    SYMHIP_CHECK(chol_.f32_updates(),
+                 "symhip: refinement residuals need SYMHIP_PRECISION_F32_UPDATES");
+    if (track >= 0) {
+      track_ = track != 0;
+      hist_.clear();
+    }
+    const int n = std::min<int>((int)hist_.size(), max);
+    if (out) std::copy(hist_.begin(), hist_.begin() + n, out);
+    return (int)hist_.size();
   }
   const double* rhs(int b) const override { return rhs_[b].get(); }
   const double* residual(int b) const override { return residual_[b].get(); }
@@ -401,6 +465,7 @@ class CscBackend final : public SystemBackend {
   int32_t M() const override { return M_; }
   int64_t nnz() const override { return nnz_; }
   SupernodalLDLT& chol() { return chol_; }
+  bool mixed() const { return chol_.f32_updates(); }
   void info(int64_t* out) const override {
     const CholSymbolic& y = chol_.symbolic();
     out[0] = y.n;
@@ -414,8 +479,134 @@ class CscBackend final : public SystemBackend {
   }
 
  private:
+  // Refinement of x = A^-1 b against the FP64 damped Hessian A: conjugate gradients
+  // preconditioned by the mixed factor, from the mixed solve's x, with FP64 products, residuals
+  // and scalars, until |b - A x| <= kRefineTol |b| (at most kRefineMax iterations). Plain
+  // refinement (x += M^-1 (b - A x)) contracts only while cond(A) times the FP32 error of the
+  // updates is below one; CG on M^-1 A still converges where it is not. Each iteration reads
+  // two scalars on the host (one small copy and a stream synchronization). CG stops at the
+  // tolerance or where it stagnates (the FP64 floor of an ill-conditioned system, ~1e-11 here
+  // for the FP64 factorization too); a breakdown, or a true residual above kFallbackTol at the
+  // end, refactors A with FP64 updates and solves directly (counted as a fallback).
+  void refine(int b, hipStream_t s) {
+    const dim3 g(grid_for(N_, 256)), blk(256);
+    double *x = x_.get(), *r = r_.get(), *z = dx_.get(), *p = p_.get(), *q = q_.get();
+    double* sc = sums_.get();  // 0: b.b, 1: r.r, 2: p.q, 3 + (k & 1): r.z of iteration k
+    auto matvec = [&](const double* rhs, const double* v, double* out) {
+      hipLaunchKernelGGL(sym_matvec_kernel, g, blk, 0, s, N_, colptr_.get(), rowind_.get(),
+                         hval_[b].get(), tptr_.get(), tcol_.get(), tidx_.get(), rhs, v, out);
+    };
+    auto read = [&]() {
+      HIP_OK(hipMemcpyAsync(host_sums_, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      return host_sums_[0] > 0 ? std::sqrt(host_sums_[1] / host_sums_[0])
+                               : std::sqrt(host_sums_[1]);
+    };
+    dot(rhs_[b].get(), rhs_[b].get(), N_, partials_.get(), sc, s);
+    matvec(rhs_[b].get(), x, r);
+    dot(r, r, N_, partials_.get(), sc + 1, s);
+    double rel = read();
+    int k = 0;
+    bool ok = !(rel > kRefineTol);
+    if (!ok) {
+      chol_.solve(r, z, s);
+      dot(r, z, N_, partials_.get(), sc + 3, s);
+      hipLaunchKernelGGL(cg_direction_kernel, g, blk, 0, s, N_, sc + 3, nullptr, z, p);
+      double best = rel;
+      int since_best = 0;
+      for (k = 1; k <= kRefineMax; ++k) {
+        double* rz = sc + 3 + ((k - 1) & 1);
+        matvec(nullptr, p, q);
+        dot(p, q, N_, partials_.get(), sc + 2, s);
+        hipLaunchKernelGGL(cg_step_kernel, g, blk, 0, s, N_, rz, sc + 2, p, q, x, r);
+        dot(r, r, N_, partials_.get(), sc + 1, s);
+        rel = read();
+        if (!(host_sums_[2] > 0) || !std::isfinite(rel)) {  // breakdown
+          ++breakdowns_;
+          break;
+        }
+        // converged, or stagnating at the FP64 floor of this system (no halving of the best
+        // residual in 3 iterations; accepted below kFallbackTol)
+        since_best = rel < 0.5 * best ? 0 : since_best + 1;
+        best = std::min(best, rel);
+        if (rel <= kRefineTol || since_best == 3) {
+          ok = true;
+          break;
+        }
+        if (k == kRefineMax) break;
+        chol_.solve(r, z, s);
+        double* rz_new = sc + 3 + (k & 1);
+        dot(r, z, N_, partials_.get(), rz_new, s);
+        hipLaunchKernelGGL(cg_direction_kernel, g, blk, 0, s, N_, rz_new, rz, z, p);
+      }
+      k = std::min(k, kRefineMax);
+      // the recurrence's residual drifts from the true one: check the true residual
+      matvec(rhs_[b].get(), x, r);
+      dot(r, r, N_, partials_.get(), sc + 1, s);
+      rel = read();
+      ok = ok && rel <= kFallbackTol;
+    }
+    if (!ok) {
+      chol_.set_f32_updates(false);
+      chol_.factorize(hval_[b].get(), s, rhs_[b].get());
+      chol_.solve_fused(x, s);
+      chol_.set_f32_updates(true);
+      matvec(rhs_[b].get(), x, r);
+      dot(r, r, N_, partials_.get(), sc + 1, s);
+      rel = read();
+      ++fallbacks_;
+    }
+    ++solves_;
+    refine_steps_ += k;
+    max_refine_steps_ = std::max<int64_t>(max_refine_steps_, k);
+    if (track_ && hist_.size() < 65536) hist_.push_back(rel);
+  }
+  static constexpr int kRefineMax = 30;
+  static constexpr double kRefineTol = 1e-13;
+  static constexpr double kFallbackTol = 1e-10;
+  // The row lists of the strictly lower entries (for sym_matvec_kernel) and the refinement
+  // work vectors; the factorization switches to FP32 updates.
+  void setup_refinement(const HostProblem& p, hipStream_t s) {
+    std::vector<int32_t> tptr(p.N + 1, 0), tcol;
+    std::vector<int64_t> tidx;
+    for (int j = 0; j < p.N; ++j)
+      for (int32_t e = p.colptr[j]; e < p.colptr[j + 1]; ++e)
+        if (p.rowind[e] > j) ++tptr[p.rowind[e] + 1];
+    for (int i = 0; i < p.N; ++i) tptr[i + 1] += tptr[i];
+    tcol.resize(std::max(1, tptr[p.N]));
+    tidx.resize(std::max(1, tptr[p.N]));
+    std::vector<int32_t> fill(tptr.begin(), tptr.end() - 1);
+    for (int j = 0; j < p.N; ++j)
+      for (int32_t e = p.colptr[j]; e < p.colptr[j + 1]; ++e)
+        if (p.rowind[e] > j) {
+          const int32_t q = fill[p.rowind[e]]++;
+          tcol[q] = j;
+          tidx[q] = e;
+        }
+    rowind_.upload(p.rowind, s);
+    tptr_.upload(tptr, s);
+    tcol_.upload(tcol, s);
+    tidx_.upload(tidx, s);
+    r_.resize(std::max(1, p.N));
+    dx_.resize(std::max(1, p.N));
+    p_.resize(std::max(1, p.N));
+    q_.resize(std::max(1, p.N));
+    sums_.resize(5);
+    HIP_OK(hipHostMalloc((void**)&host_sums_, 5 * sizeof(double), hipHostMallocDefault));
+    chol_.set_f32_updates(true);
+    HIP_OK(hipStreamSynchronize(s));
+  }
+
   CscLinearizer lin_;
   SupernodalLDLT chol_;
+  DBuf<double> r_, dx_, p_, q_, sums_;
+  double* host_sums_ = nullptr;  // pinned
+  DBuf<int32_t> rowind_, tptr_, tcol_;
+  DBuf<int64_t> tidx_;
+  int64_t solves_ = 0, refine_steps_ = 0, max_refine_steps_ = 0, fallbacks_ = 0;
+  int64_t breakdowns_ = 0;  // CG breakdowns among the fallbacks
+  bool track_ = false;
+  std::vector<double> hist_;
   DBuf<double> residual_[3], hval_[3], rhs_[3];
   DBuf<double> undamped_, damping_, max_diag_, x_, partials_;
   DBuf<int32_t> colptr_;
@@ -438,9 +629,13 @@ DeviceOptimizer::DeviceOptimizer(const symhip_problem_desc* d, const symhip_opti
   // Optimizer ctor (optimizer.tcc:39): the derivative check compares against the Jacobian
   SYMHIP_CHECK(!p.check_derivatives || p.include_jacobians,
                "check_derivatives requires include_jacobians");
-  SYMHIP_CHECK(o.precision == SYMHIP_PRECISION_F64 ||
-                   (o.precision == SYMHIP_PRECISION_F32_JAC && o.linear_solver == kSolverSchur),
+  SYMHIP_CHECK(o.precision != SYMHIP_PRECISION_F32_JAC || o.linear_solver == kSolverSchur,
                "symhip: SYMHIP_PRECISION_F32_JAC is available on the Schur path only");
+  SYMHIP_CHECK(o.precision != SYMHIP_PRECISION_F32_UPDATES || o.linear_solver != kSolverSchur,
+               "symhip: SYMHIP_PRECISION_F32_UPDATES is available on the sparse Cholesky path "
+               "only");
+  SYMHIP_CHECK(o.precision >= SYMHIP_PRECISION_F64 && o.precision <= SYMHIP_PRECISION_F32_UPDATES,
+               "symhip: unknown precision");
   SYMHIP_CHECK(!o.use_pcg || o.linear_solver == kSolverSchur,
                "symhip: PCG is available on the Schur path only");
   build_host_problem(d, host_);
