@@ -20,9 +20,13 @@ Prints one JSON line (value = LM iterations/s of the timed FP64 run) with:
                 reference orders it with METIS; also the -ffast-math build of the same sources
   batch         4096 robot-3D-localization problems in one persistent launch (FP64 and FP32
                 Jacobian forms), against the CPU oracle on one core and on all cores
-  pose_graph    the 100 000-node pose graph (Cholesky path): LM iterations/s, phases, run to
-                convergence, the factorization phase's roofline, and the CPU oracle on the same
-                problem and elimination order
+  pose_graph    the 100 000-node pose graph (Cholesky path): LM iterations/s, phases, the
+                mixed-precision factorization form, run to convergence, the factorization
+                phase's roofline, and the CPU oracle on the same problem and elimination order
+  ba5k          the 5000-camera bundle adjustment (~5M observations): LM iterations/s and phases,
+                the run against the CPU oracle's recorded run (tests/golden/ba5k_oracle.json),
+                and strict PCG with block Jacobi and with the learned correction trained on the
+                1000-camera problem's systems (iterations, fallbacks, final-cost gap)
 """
 from __future__ import annotations
 
@@ -306,6 +310,24 @@ def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
     opt.optimize_resident(num_iterations=steps)
     phases = {k: v[0] / max(1, v[1]) for k, v in opt.phase_times().items() if v[1]}
     opt.enable_timing(False)
+    # the mixed-precision form of the factorization (FP32 matrix-core updates, CG refinement)
+    mopt = capi.HipOptimizer(prob, lm_params(steps + 2, early_exit=0.0), capi.solver_options(
+        linear_solver=capi.SOLVER_CHOLESKY, ordering=capi.ORDERING_NESTED_DISSECTION,
+        precision=capi.PRECISION_F32_UPDATES))
+    mopt.set_values(prob.values)
+    mopt.optimize_resident(num_iterations=2)
+    t0 = time.perf_counter()
+    mopt.optimize_resident(num_iterations=steps)
+    mixed_ms = 1e3 * (time.perf_counter() - t0) / steps
+    mopt.enable_timing(True)
+    mopt.optimize_resident(num_iterations=steps)
+    mixed = {"ms_per_iter": mixed_ms,
+             "phases_ms_per_iter": {k: v[0] / max(1, v[1])
+                                    for k, v in mopt.phase_times().items() if v[1]},
+             "linear_solver": mopt.linear_solver_stats(),
+             "form": "FP32 matrix-core trailing updates, FP64 diagonal blocks and solves, CG "
+                     "refinement against the FP64 Hessian"}
+    del mopt
     conv = capi.HipOptimizer(prob, capi.default_params(iterations=100), opts)
     conv.set_values(prob.values)
     conv.optimize_resident()  # warm
@@ -316,6 +338,7 @@ def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
     out = {"workload": "pose_graph_100k_nodes_115k_edges_cauchy_loops",
            "n_loops": int(prob.info["n_loops"]), "unknowns": int(info["n"]),
            "lm_iters_per_s": 1e3 / ms, "ms_per_iter": ms, "phases_ms_per_iter": phases,
+           "mixed_precision": mixed,
            "converge": {"s": t_conv, "iterations": len(res["iterations"]) - 1,
                         "status": res["status"],
                         "final_cost": res["iterations"][res["best_index"]]["new_error"]},
@@ -350,32 +373,71 @@ def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
     return out
 
 
-def schur_elimination_order(prob):
-    """Elimination order of the reduced camera system for the CPU oracle: nested dissection of
-    the camera covisibility graph (the reference orders S with METIS, sparse_schur_solver.h:77),
-    each camera's 6 pose and 3 intrinsics columns kept together. State order is all poses, then
-    all intrinsics, then the points."""
-    ft, fp, fa = (np.asarray(x) for x in (prob.ftypes, prob.fptr, prob.fargs))
-    starts = fp[:-1][ft == capi.FACTOR_SNAVELY_HUBER]
-    cam_key, pt_key = fa[starts], fa[starts + 2]
-    ucam = np.unique(cam_key)
-    cam = np.searchsorted(ucam, cam_key)
-    o = np.lexsort((cam, pt_key))
-    pk, ck = pt_key[o], cam[o]
-    n = ucam.size
-    rows, cols = [np.arange(n)], [np.arange(n)]
-    for d in range(1, int(np.bincount(pk).max())):
-        same = pk[d:] == pk[:-d]
-        a, b = ck[:-d][same], ck[d:][same]
-        rows.append(np.maximum(a, b))
-        cols.append(np.minimum(a, b))
-    key = np.unique(np.concatenate(cols).astype(np.int64) * n + np.concatenate(rows))
-    c, r = key // n, key % n
-    colptr = np.concatenate([[0], np.cumsum(np.bincount(c, minlength=n))]).astype(np.int32)
-    order = capi.HipCholesky(n, colptr, r.astype(np.int32),
-                             ordering=capi.ORDERING_NESTED_DISSECTION).inv_permutation()
-    return np.concatenate([np.concatenate([6 * k + np.arange(6), 6 * n + 3 * k + np.arange(3)])
-                           for k in order]).astype(np.int32)
+def ba5k_leg(train_prob, steps=10):
+    """Bundle adjustment at 5000 cameras (1 000 000 points, ~5M observations, Huber): the plain
+    FP64 Schur path timed for `steps` LM iterations with phase timing and run to its end against
+    the CPU oracle's recorded run (tests/golden/ba5k_oracle.json: the reference LM ends FAILED,
+    lambda out of bounds, after 14 iterations), then PCG on the reduced camera system with block
+    Jacobi and with the learned correction trained on systems logged from `train_prob` (the
+    1000-camera problem), strict (the LDL^T solve when a solve misses 1e-8): wall time, PCG
+    iterations, fallbacks and the final-cost gap to the oracle."""
+    prob = capi.gen_ba_snavely(5000, 1000000, 42)
+    exact = capi.solver_options(linear_solver=capi.SOLVER_SCHUR,
+                                ordering=capi.ORDERING_NESTED_DISSECTION)
+    fixture = REPO / "tests" / "golden" / "ba5k_oracle.json"
+    target = json.loads(fixture.read_text())["final_cost"] if fixture.exists() else None
+    t0 = time.time()
+    opt = capi.HipOptimizer(prob, lm_params(steps + 2, early_exit=0.0), exact)
+    t_setup = time.time() - t0
+    info = opt.solver_info()
+    opt.set_values(prob.values)
+    opt.optimize_resident(num_iterations=2)
+    t0 = time.perf_counter()
+    opt.optimize_resident(num_iterations=steps)
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    opt.enable_timing(True)
+    opt.optimize_resident(num_iterations=steps)
+    phases = {k: v[0] / max(1, v[1]) for k, v in opt.phase_times().items() if v[1]}
+    del opt
+
+    def run(opts, params=None):
+        o = capi.HipOptimizer(prob, lm_params(100), opts)
+        if params is not None:
+            o.preconditioner_params(params)
+        t0 = time.perf_counter()
+        res = o.optimize_resident()
+        dt = time.perf_counter() - t0
+        best = res["iterations"][res["best_index"]]["new_error"]
+        return {"s": dt, "iterations": len(res["iterations"]) - 1, "status": res["status"],
+                "final_cost": best, "linear_solver": o.linear_solver_stats(),
+                "rel_gap": None if target is None else (best - target) / target}
+
+    out = {"workload": "ba_snavely_5000cams_1Mpts_5Mobs_huber_schur",
+           "n_obs": int(prob.info["n_obs"]), "unknowns_rcs": int(info["n"]),
+           "setup_s": t_setup, "ms_per_iter": ms, "lm_iters_per_s": 1e3 / ms,
+           "phases_ms_per_iter": phases, "oracle_final_cost": target,
+           "plain": run(exact)}
+    trainer = capi.HipOptimizer(train_prob, capi.default_params(
+        lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=12), capi.solver_options(
+        linear_solver=capi.SOLVER_SCHUR, ordering=capi.ORDERING_NESTED_DISSECTION, use_pcg=1,
+        pcg_max_iters=300, pcg_rel_tol=1e-6, pcg_preconditioner=capi.PRECOND_LEARNED))
+    trainer.log_systems(8)
+    trainer.optimize()
+    t0 = time.perf_counter()
+    losses = trainer.train_preconditioner(300, lr=1e-3, probes=4, seed=11)
+    t_train = time.perf_counter() - t0
+    params = trainer.preconditioner_params()
+    del trainer
+    for name, pre in (("pcg_block_jacobi", capi.PRECOND_BLOCK_JACOBI),
+                      ("pcg_learned", capi.PRECOND_LEARNED)):
+        out[name] = run(capi.solver_options(
+            linear_solver=capi.SOLVER_SCHUR, ordering=capi.ORDERING_NESTED_DISSECTION, use_pcg=2,
+            pcg_max_iters=300, pcg_rel_tol=1e-8, pcg_preconditioner=pre), params)
+    out["pcg_learned"]["training"] = {"systems": 8, "steps": 300, "s": t_train,
+                                      "loss_first": float(losses[0]),
+                                      "loss_last": float(losses[-1]),
+                                      "source": "1000-camera problem's own solves"}
+    return out
 
 
 def cpu_baseline(prob, iters=3):
@@ -385,7 +447,7 @@ def cpu_baseline(prob, iters=3):
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
 
-    order = schur_elimination_order(prob)
+    order = ow.schur_elimination_order(prob)
     model, _ = host_cpu()
     res = {}
     for name, fast in (("plain", False), ("fast_math", True)):
@@ -434,6 +496,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
+    ap.add_argument("--no-ba5k", action="store_true", help="skip the 5000-camera leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
     ap.add_argument("--no-pose-graph", action="store_true",
                     help="skip the 100k-node pose-graph leg")
@@ -507,6 +570,8 @@ def main():
         out["batch"] = batch_leg()
     if not args.no_pose_graph:
         out["pose_graph"] = pose_graph_leg()
+    if not args.no_ba5k:
+        out["ba5k"] = ba5k_leg(r["prob"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(r["prob"])
     print(json.dumps(out))

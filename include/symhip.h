@@ -47,6 +47,15 @@ int symhip_problem_state_keys(symhip_problem_t problem, int32_t* key_index, int3
    linearized arguments (the first ones; the rest are constants), out[2] = residual dimension,
    out[3] = tangent dimension of the linearized arguments. */
 int symhip_factor_info(int32_t code, int32_t* out4);
+/* One registered factor body evaluated on the host (what the device kernels run): residual (D)
+   and Jacobian (D x T, row-major, over the linearized arguments' tangents in argument order) at
+   the argument storage pointers args (one per argument, as symhip_problem_desc lists them).
+   precision SYMHIP_PRECISION_F64 runs the FP64 body, SYMHIP_PRECISION_F32_JAC the FP32 body of
+   the precision forms (relative rotations, camera-frame points and measured-minus-predicted
+   terms formed in FP64 and rounded once, the rest in FP32). No reference counterpart (the
+   reference evaluates generated functions in its Scalar); for checks of the bodies. */
+int symhip_factor_linearize_host(int32_t code, const double* const* args, int32_t precision,
+                                 double* residual, double* jacobian);
 
 /* Linearizer::Relinearize (opt/linearizer.cc:54-120) at the given Values storage. Host buffers in
    and out (copies over PCIe; the optimizer keeps everything resident instead). Outputs may be

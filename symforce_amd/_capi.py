@@ -481,6 +481,21 @@ _EXTRA_SIGNATURES: list = [
 ]
 
 
+def factor_linearize_host(ftype, args, precision=PRECISION_F64):
+    """symhip_factor_linearize_host: one registered factor body on the host; args is the list of
+    argument storage arrays. Returns (residual (D,), Jacobian (D, T))."""
+    info = np.zeros(4, np.int32)
+    check(hip_lib().symhip_factor_info(int(ftype), ptr(info, C.c_int32)))
+    arrs = [np.ascontiguousarray(a, np.float64) for a in args]
+    assert len(arrs) == info[0], "argument count"
+    ptrs = (F64P * len(arrs))(*[a.ctypes.data_as(F64P) for a in arrs])
+    r = np.zeros(info[2])
+    J = np.zeros(info[2] * info[3])
+    check(hip_lib().symhip_factor_linearize_host(int(ftype), ptrs, precision,
+                                                 ptr(r, C.c_double), ptr(J, C.c_double)))
+    return r, J.reshape(info[2], info[3])
+
+
 def check(rc):
     if rc != 0:
         raise RuntimeError(hip_lib().symhip_last_error().decode())

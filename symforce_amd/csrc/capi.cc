@@ -108,6 +108,31 @@ int symhip_factor_info(int32_t code, int32_t* out) {
   });
 }
 
+int symhip_factor_linearize_host(int32_t code, const double* const* args, int32_t precision,
+                                 double* residual, double* jacobian) {
+  return guarded([&] {
+    SYMHIP_CHECK(args && residual && jacobian, "invalid arguments");
+    SYMHIP_CHECK(precision == SYMHIP_PRECISION_F64 || precision == SYMHIP_PRECISION_F32_JAC,
+                 "symhip: precision must be SYMHIP_PRECISION_F64 or SYMHIP_PRECISION_F32_JAC");
+    symhip::dispatch_factor_type(code, [&]<int C>() {
+      using F = symhip::FT<C>;
+      auto run = [&](auto zero) {
+        using S = decltype(zero);
+        S r[F::D], J[F::D][F::T];
+        F::template eval<S>(args, r, J);
+        for (int i = 0; i < F::D; ++i) {
+          residual[i] = double(r[i]);
+          for (int c = 0; c < F::T; ++c) jacobian[i * F::T + c] = double(J[i][c]);
+        }
+      };
+      if (precision == SYMHIP_PRECISION_F64)
+        run(0.0);
+      else
+        run(0.0f);
+    });
+  });
+}
+
 int symhip_problem_destroy(symhip_problem_t p) {
   return guarded([&] { delete p; });
 }

@@ -155,6 +155,25 @@ SYMHIP_HD V3<S> load_v3(const double* p) {
   return {S(p[0]), S(p[1]), S(p[2])};
 }
 
+// Relative Lie-group deltas for the FP32 bodies of the precision forms: the quantities that
+// cancel -- a relative rotation q_a^-1 q_b, a measured-minus-predicted rotation or translation, a
+// point in a camera frame -- are formed in FP64 from the FP64 Values and rounded once, so FP32
+// rounding enters after the cancellation (each keeps FP32's relative accuracy instead of losing
+// the ratio of absolute to relative magnitude, e.g. a 1 km pose and a 1 m step). With S =
+// double these are the FP64 body's own expressions.
+template <typename S>
+SYMHIP_HD Quat<S> to_s(const Quat<double>& q) {
+  return {S(q.x), S(q.y), S(q.z), S(q.w)};
+}
+template <typename S>
+SYMHIP_HD V3<S> to_s(const V3<double>& v) {
+  return {S(v.x), S(v.y), S(v.z)};
+}
+// q_a^-1 q_b in FP64
+SYMHIP_HD Quat<double> rel_quat(const double* pa, const double* pb) {
+  return qmul(qconj(load_quat<double>(pa)), load_quat<double>(pb));
+}
+
 // Retraction exponential [sin(t/2)/t w, cos(t/2)], t = sqrt(eps^2 + |w|^2).
 SYMHIP_HD void rot3_retract_inplace(double* q, const double* w, double eps) {
   const double t = sqrt(eps * eps + w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
@@ -236,10 +255,9 @@ SYMHIP_HD void whiten_norm(int kind, S param, S eps, S* r, S (*J)[T]) {
 // local(prior, value) rotation part, perturbing `value` (q_rel = prior^-1 value).
 template <typename S>
 SYMHIP_HD void factor_prior_pose3(const double* const* a, S* r, S (*J)[6]) {
-  const Quat<S> qv = load_quat<S>(a[0]), qp = load_quat<S>(a[1]);
   const double* Sm = a[2];
   const S eps = S(a[3][0]);
-  const Quat<S> qrel = qmul(qconj(qp), qv);
+  const Quat<S> qrel = to_s<S>(rel_quat(a[1], a[0]));
   V3<S> w;
   M3<S> D;
   rot_log(qrel, eps, w, D);
@@ -264,21 +282,22 @@ SYMHIP_HD void factor_prior_pose3(const double* const* a, S* r, S (*J)[6]) {
 template <typename S>
 SYMHIP_HD void between_pose3_error(const double* pa, const double* pb, const double* pm, S eps,
                                    S* e, S (*de)[12]) {
-  const Quat<S> qa = load_quat<S>(pa), qb = load_quat<S>(pb), qm = load_quat<S>(pm);
-  const Quat<S> qab = qmul(qconj(qa), qb);
-  const M3<S> Ra = rotmat(qa), Rab = rotmat(qab);
-  const V3<S> dt{S(pb[4] - pa[4]), S(pb[5] - pa[5]), S(pb[6] - pa[6])};
-  const V3<S> tab = mulT(Ra, dt);
-  const Quat<S> qrel = qmul(qconj(qm), qab);
+  const Quat<double> qab_d = rel_quat(pa, pb);
+  const V3<double> dt{pb[4] - pa[4], pb[5] - pa[5], pb[6] - pa[6]};
+  const V3<double> tab_d = mulT(rotmat(load_quat<double>(pa)), dt);
+  const Quat<S> qab = to_s<S>(qab_d);
+  const M3<S> Ra = rotmat(load_quat<S>(pa)), Rab = rotmat(qab);
+  const V3<S> tab = to_s<S>(tab_d);
+  const Quat<S> qrel = to_s<S>(qmul(qconj(load_quat<double>(pm)), qab_d));
   V3<S> w;
   M3<S> D;
   rot_log(qrel, eps, w, D);
   e[0] = w.x;
   e[1] = w.y;
   e[2] = w.z;
-  e[3] = tab.x - S(pm[4]);
-  e[4] = tab.y - S(pm[5]);
-  e[5] = tab.z - S(pm[6]);
+  e[3] = S(tab_d.x - pm[4]);
+  e[4] = S(tab_d.y - pm[5]);
+  e[5] = S(tab_d.z - pm[6]);
   // rotation rows: d/dth_a = D (-Rab^T), d/dth_b = D
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
@@ -321,7 +340,7 @@ SYMHIP_HD void factor_between_pose3(const double* const* a, S* r, S (*J)[12], in
 
 template <typename S>
 SYMHIP_HD void factor_prior_rot3(const double* const* a, S* r, S (*J)[3]) {
-  const Quat<S> qrel = qmul(qconj(load_quat<S>(a[1])), load_quat<S>(a[0]));
+  const Quat<S> qrel = to_s<S>(rel_quat(a[1], a[0]));
   V3<S> w;
   M3<S> D;
   rot_log(qrel, S(a[3][0]), w, D);
@@ -341,12 +360,11 @@ SYMHIP_HD void factor_prior_rot3(const double* const* a, S* r, S (*J)[3]) {
 
 template <typename S>
 SYMHIP_HD void factor_between_rot3(const double* const* a, S* r, S (*J)[6]) {
-  const Quat<S> qa = load_quat<S>(a[0]), qb = load_quat<S>(a[1]), qm = load_quat<S>(a[2]);
-  const Quat<S> qab = qmul(qconj(qa), qb);
-  const M3<S> Rab = rotmat(qab);
+  const Quat<double> qab_d = rel_quat(a[0], a[1]);
+  const M3<S> Rab = rotmat(to_s<S>(qab_d));
   V3<S> w;
   M3<S> D;
-  rot_log(qmul(qconj(qm), qab), S(a[4][0]), w, D);
+  rot_log(to_s<S>(qmul(qconj(load_quat<double>(a[2])), qab_d)), S(a[4][0]), w, D);
   const S e[3] = {w.x, w.y, w.z};
   S de[3][6];
   for (int i = 0; i < 3; ++i)
@@ -373,12 +391,12 @@ SYMHIP_HD void factor_between_rot3(const double* const* a, S* r, S (*J)[6]) {
 // J over [dth(3) dt(3) | f k1 k2 | X(3)].
 template <typename S>
 SYMHIP_HD void factor_snavely(const double* const* a, S* r, S (*J)[12], int robust) {
-  const Quat<S> q = load_quat<S>(a[0]);
-  const M3<S> R = rotmat(q);
+  const M3<S> R = rotmat(load_quat<S>(a[0]));
   const V3<S> X = load_v3<S>(a[2]);
-  const V3<S> t{S(a[0][4]), S(a[0][5]), S(a[0][6])};
   const S f = S(a[1][0]), k1 = S(a[1][1]), k2 = S(a[1][2]);
-  const V3<S> P = mul(R, X) + t;
+  // the point in the camera frame, formed in FP64
+  const V3<S> P = to_s<S>(mul(rotmat(load_quat<double>(a[0])), load_v3<double>(a[2])) +
+                          V3<double>{a[0][4], a[0][5], a[0][6]});
   const S iz = S(1) / P.z;
   const S px = -P.x * iz, py = -P.y * iz;
   const S r2 = px * px + py * py;
@@ -417,11 +435,11 @@ SYMHIP_HD void factor_snavely(const double* const* a, S* r, S (*J)[12], int robu
 template <typename S>
 SYMHIP_HD void factor_reproj_cauchy(const double* const* a, S* r, S (*J)[9]) {
   const M3<S> R = rotmat(load_quat<S>(a[0]));
-  const V3<S> t{S(a[0][4]), S(a[0][5]), S(a[0][6])};
-  const V3<S> X = load_v3<S>(a[1]);
   const double* K = a[3];
   const S eps = S(a[5][0]);
-  const V3<S> P = mulT(R, X - t);
+  // the point in the camera frame, formed in FP64
+  const V3<S> P = to_s<S>(mulT(rotmat(load_quat<double>(a[0])),
+                               load_v3<double>(a[1]) - V3<double>{a[0][4], a[0][5], a[0][6]}));
   const bool zin = P.z > eps;
   const S z = zin ? P.z : eps;
   const S fx = S(K[0]), fy = S(K[1]);
@@ -445,12 +463,14 @@ SYMHIP_HD void factor_reproj_cauchy(const double* const* a, S* r, S (*J)[9]) {
 template <typename S>
 SYMHIP_HD void factor_matching(const double* const* a, S* r, S (*J)[6]) {
   const M3<S> R = rotmat(load_quat<S>(a[0]));
-  const V3<S> t{S(a[0][4]), S(a[0][5]), S(a[0][6])};
-  const V3<S> pred = mulT(R, load_v3<S>(a[1]) - t);
+  // the landmark in the pose frame and its difference to the measurement, formed in FP64
+  const V3<double> pred_d = mulT(rotmat(load_quat<double>(a[0])),
+                                 load_v3<double>(a[1]) - V3<double>{a[0][4], a[0][5], a[0][6]});
+  const V3<S> pred = to_s<S>(pred_d);
   const S inv = S(1) / S(a[3][0]);
-  r[0] = (pred.x - S(a[2][0])) * inv;
-  r[1] = (pred.y - S(a[2][1])) * inv;
-  r[2] = (pred.z - S(a[2][2])) * inv;
+  r[0] = S(pred_d.x - a[2][0]) * inv;
+  r[1] = S(pred_d.y - a[2][1]) * inv;
+  r[2] = S(pred_d.z - a[2][2]) * inv;
   const M3<S> PK = skew(pred);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
@@ -462,18 +482,19 @@ SYMHIP_HD void factor_matching(const double* const* a, S* r, S (*J)[6]) {
 // Odometry: pred = a^-1 b; e = local(pred, meas) = [log(pred^-1 meas), t_m - t_pred]; r = e/sigma.
 template <typename S>
 SYMHIP_HD void factor_odometry(const double* const* a, S* r, S (*J)[12]) {
-  const Quat<S> qa = load_quat<S>(a[0]), qb = load_quat<S>(a[1]), qm = load_quat<S>(a[2]);
-  const Quat<S> qab = qmul(qconj(qa), qb);
-  const M3<S> Ra = rotmat(qa), Rab = rotmat(qab);
-  const V3<S> dt{S(a[1][4] - a[0][4]), S(a[1][5] - a[0][5]), S(a[1][6] - a[0][6])};
-  const V3<S> tab = mulT(Ra, dt);
-  const Quat<S> qrel = qmul(qconj(qab), qm);
+  const Quat<double> qab_d = rel_quat(a[0], a[1]);
+  const V3<double> dt{a[1][4] - a[0][4], a[1][5] - a[0][5], a[1][6] - a[0][6]};
+  const V3<double> tab_d = mulT(rotmat(load_quat<double>(a[0])), dt);
+  const M3<S> Ra = rotmat(load_quat<S>(a[0])), Rab = rotmat(to_s<S>(qab_d));
+  const V3<S> tab = to_s<S>(tab_d);
+  const Quat<S> qrel = to_s<S>(qmul(qconj(qab_d), load_quat<double>(a[2])));
   const M3<S> Rrel = rotmat(qrel);
   V3<S> w;
   M3<S> D;
   rot_log(qrel, S(a[4][0]), w, D);
   const double* sig = a[3];
-  const S e[6] = {w.x, w.y, w.z, S(a[2][4]) - tab.x, S(a[2][5]) - tab.y, S(a[2][6]) - tab.z};
+  const S e[6] = {w.x, w.y, w.z, S(a[2][4] - tab_d.x), S(a[2][5] - tab_d.y),
+                  S(a[2][6] - tab_d.z)};
   // rotation: d/dth_a = D Rrel^T Rab^T ; d/dth_b = -D Rrel^T
   const M3<S> RrT = transpose(Rrel);
   const M3<S> G = matmul(RrT, transpose(Rab));

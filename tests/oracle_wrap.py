@@ -280,3 +280,31 @@ def csc_to_dense(n, colptr, rowind, vals, rows=None):
         for q in range(colptr[c], colptr[c + 1]):
             A[rowind[q], c] += vals[q]
     return A
+
+
+def schur_elimination_order(prob):
+    """Elimination order of the reduced camera system for the CPU oracle: nested dissection of
+    the camera covisibility graph (the reference orders S with METIS, sparse_schur_solver.h:77),
+    each camera's 6 pose and 3 intrinsics columns kept together. State order is all poses, then
+    all intrinsics, then the points."""
+    ft, fp, fa = (np.asarray(x) for x in (prob.ftypes, prob.fptr, prob.fargs))
+    starts = fp[:-1][ft == capi.FACTOR_SNAVELY_HUBER]
+    cam_key, pt_key = fa[starts], fa[starts + 2]
+    ucam = np.unique(cam_key)
+    cam = np.searchsorted(ucam, cam_key)
+    o = np.lexsort((cam, pt_key))
+    pk, ck = pt_key[o], cam[o]
+    n = ucam.size
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for d in range(1, int(np.bincount(pk).max())):
+        same = pk[d:] == pk[:-d]
+        a, b = ck[:-d][same], ck[d:][same]
+        rows.append(np.maximum(a, b))
+        cols.append(np.minimum(a, b))
+    key = np.unique(np.concatenate(cols).astype(np.int64) * n + np.concatenate(rows))
+    c, r = key // n, key % n
+    colptr = np.concatenate([[0], np.cumsum(np.bincount(c, minlength=n))]).astype(np.int32)
+    order = capi.HipCholesky(n, colptr, r.astype(np.int32),
+                             ordering=capi.ORDERING_NESTED_DISSECTION).inv_permutation()
+    return np.concatenate([np.concatenate([6 * k + np.arange(6), 6 * n + 3 * k + np.arange(3)])
+                           for k in order]).astype(np.int32)
