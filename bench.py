@@ -290,7 +290,29 @@ def batch_leg(n_problems=4096, cpu_sample=64):
     return out
 
 
-def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
+def mixed_precision_pose_graph(prob, steps):
+    """The mixed-precision form of the pose graph's factorization (FP32 matrix-core trailing
+    updates, FP64 diagonal blocks and solves, CG refinement against the FP64 Hessian): LM
+    iterations timed and phase-timed like the FP64 form."""
+    mopt = capi.HipOptimizer(prob, lm_params(steps + 2, early_exit=0.0), capi.solver_options(
+        linear_solver=capi.SOLVER_CHOLESKY, ordering=capi.ORDERING_NESTED_DISSECTION,
+        precision=capi.PRECISION_F32_UPDATES))
+    mopt.set_values(prob.values)
+    mopt.optimize_resident(num_iterations=2)
+    t0 = time.perf_counter()
+    mopt.optimize_resident(num_iterations=steps)
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    mopt.enable_timing(True)
+    mopt.optimize_resident(num_iterations=steps)
+    return {"ms_per_iter": ms,
+            "phases_ms_per_iter": {k: v[0] / max(1, v[1])
+                                   for k, v in mopt.phase_times().items() if v[1]},
+            "linear_solver": mopt.linear_solver_stats(),
+            "form": "FP32 matrix-core trailing updates, FP64 diagonal blocks and solves, CG "
+                    "refinement against the FP64 Hessian"}
+
+
+def pose_graph_leg(steps=10, cpu_iters=5, cpu=True, mixed=True):
     """Pose-graph SLAM at 100 000 Pose3 nodes (115 037 between factors, 15 037 of them Cauchy
     loop closures): the sparse Cholesky path (supernodal LDL^T over the nested-dissection
     order) timed for `steps` LM iterations with phase timing, the factorization phase against
@@ -311,23 +333,7 @@ def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
     phases = {k: v[0] / max(1, v[1]) for k, v in opt.phase_times().items() if v[1]}
     opt.enable_timing(False)
     # the mixed-precision form of the factorization (FP32 matrix-core updates, CG refinement)
-    mopt = capi.HipOptimizer(prob, lm_params(steps + 2, early_exit=0.0), capi.solver_options(
-        linear_solver=capi.SOLVER_CHOLESKY, ordering=capi.ORDERING_NESTED_DISSECTION,
-        precision=capi.PRECISION_F32_UPDATES))
-    mopt.set_values(prob.values)
-    mopt.optimize_resident(num_iterations=2)
-    t0 = time.perf_counter()
-    mopt.optimize_resident(num_iterations=steps)
-    mixed_ms = 1e3 * (time.perf_counter() - t0) / steps
-    mopt.enable_timing(True)
-    mopt.optimize_resident(num_iterations=steps)
-    mixed = {"ms_per_iter": mixed_ms,
-             "phases_ms_per_iter": {k: v[0] / max(1, v[1])
-                                    for k, v in mopt.phase_times().items() if v[1]},
-             "linear_solver": mopt.linear_solver_stats(),
-             "form": "FP32 matrix-core trailing updates, FP64 diagonal blocks and solves, CG "
-                     "refinement against the FP64 Hessian"}
-    del mopt
+    mixed_out = mixed and mixed_precision_pose_graph(prob, steps)
     conv = capi.HipOptimizer(prob, capi.default_params(iterations=100), opts)
     conv.set_values(prob.values)
     conv.optimize_resident()  # warm
@@ -338,7 +344,7 @@ def pose_graph_leg(steps=10, cpu_iters=5, cpu=True):
     out = {"workload": "pose_graph_100k_nodes_115k_edges_cauchy_loops",
            "n_loops": int(prob.info["n_loops"]), "unknowns": int(info["n"]),
            "lm_iters_per_s": 1e3 / ms, "ms_per_iter": ms, "phases_ms_per_iter": phases,
-           "mixed_precision": mixed,
+           "mixed_precision": mixed_out or None,
            "converge": {"s": t_conv, "iterations": len(res["iterations"]) - 1,
                         "status": res["status"],
                         "final_cost": res["iterations"][res["best_index"]]["new_error"]},
@@ -497,6 +503,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-problem leg")
     ap.add_argument("--no-ba5k", action="store_true", help="skip the 5000-camera leg")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="pose-graph leg without the mixed-precision factorization form")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-form legs")
     ap.add_argument("--no-pose-graph", action="store_true",
                     help="skip the 100k-node pose-graph leg")
@@ -520,7 +528,7 @@ def main():
         calibrate_copy()
         return
     if args.pose_graph_only:
-        print(json.dumps(pose_graph_leg(cpu=False)))
+        print(json.dumps(pose_graph_leg(cpu=False, mixed=not args.no_mixed)))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the optimizer path runs on a single GPU "

@@ -190,7 +190,10 @@ typedef struct symhip_solver_options {
   double pcg_rel_tol;
   int32_t pcg_preconditioner; /* SYMHIP_PRECOND_BLOCK_JACOBI or SYMHIP_PRECOND_LEARNED */
   /* Truncated PCG only: once an accepted LM step's relative reduction falls below this value the
-     remaining iterations of the run use the direct LDL^T (0: never). */
+     remaining iterations of the run use the direct LDL^T (0: never). Until then the LM early
+     exit on a small relative reduction is deferred: a truncated step that meets it switches the
+     run to the direct LDL^T instead of ending it (a stalled inexact solve is not convergence).
+     At the switch lambda is relaxed to min(lambda, initial_lambda) (and nu to its beta). */
   double pcg_exact_below_reduction;
   /* Truncated PCG only, inexact-Newton forcing (0: off): after an accepted LM step with relative
      reduction r the next solve uses tolerance min(pcg_rel_tol, pcg_forcing * sqrt(r)) (at least

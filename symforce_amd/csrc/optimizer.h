@@ -72,6 +72,12 @@ class SystemBackend {
   virtual int refinement_residuals(int track, double* out, int max, hipStream_t s) {
     throw Error("symhip: refinement residuals need SYMHIP_PRECISION_F32_UPDATES");
   }
+  // True while the solver's steps are inexact and it will switch to exact ones (truncated PCG
+  // with an exact switch): a small relative reduction then signals the switch (observe), not
+  // convergence, and the LM's early exit on it is deferred to the exact steps.
+  virtual bool defers_early_exit() const { return false; }
+  // Called when a deferred early exit would have ended the run: switch to the exact steps.
+  virtual void switch_to_exact() {}
   // Called at the start of every optimize run and after every LM iteration record.
   virtual void begin_run() {}
   virtual void observe(const symhip_iteration_stats& rec) { (void)rec; }

@@ -1108,8 +1108,13 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
     dot(update_.get(), update_.get(), N, partials_.get(), &dstate_->nu2, stream_);
     dot(last_update_.get(), update_.get(), N, partials_.get(), &dstate_->dotlu, stream_);
   }
+  // An inexact (truncated PCG) step whose reduction falls below the early-exit threshold switches
+  // the solver to exact steps instead of ending the run; the early exit applies to those.
+  symhip_optimizer_params pd = p_;
+  const bool deferred = backend_->defers_early_exit();
+  if (deferred) pd.early_exit_min_reduction = 0.0;
   hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(1), 0, stream_, dstate_, init_, new_, best_,
-                     iteration_, p_, have_last_update_ ? 1 : 0, epsilon_);
+                     iteration_, pd, have_last_update_ ? 1 : 0, epsilon_);
   timer_.end(decide_ev, stream_);
   HIP_OK(hipMemcpyAsync(hstate_, dstate_, sizeof(LmDevState), hipMemcpyDeviceToHost, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
@@ -1119,6 +1124,16 @@ bool DeviceOptimizer::iterate(std::vector<symhip_iteration_stats>& iters, int32_
   check_iteration();
   if (p_.debug_stats) record_debug(new_, true);
   backend_->observe(hstate_->rec);
+  if (deferred) {
+    const double rel = hstate_->rec.relative_reduction, ee = p_.early_exit_min_reduction;
+    if (rel > -ee / 10 && rel < ee) backend_->switch_to_exact();
+    // At the switch the damping that made the inexact steps acceptable is relaxed to the initial
+    // one (lambda = min(lambda, initial), nu = beta): it says nothing about exact steps, and a
+    // lambda grown there would make the first exact steps small enough for the early exit.
+    if (!backend_->defers_early_exit())
+      hipLaunchKernelGGL(lm_relax_kernel, dim3(1), dim3(1), 0, stream_, dstate_,
+                         p_.initial_lambda, p_.dynamic_lambda_update_beta);
+  }
   if (!hstate_->accept) {
     std::swap(new_, init_);  // SwapNewAndInit
   } else {

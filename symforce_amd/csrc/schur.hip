@@ -751,6 +751,10 @@ class SchurBackend final : public SystemBackend {
     exact_now_ = false;
     last_reduction_ = -1;
   }
+  bool defers_early_exit() const override {
+    return use_pcg_ && !pcg_strict_ && exact_below_ > 0 && !exact_now_;
+  }
+  void switch_to_exact() override { exact_now_ = true; }
   void observe(const symhip_iteration_stats& rec) override {
     if (!use_pcg_ || !rec.update_accepted || rec.relative_reduction < 0) return;
     last_reduction_ = rec.relative_reduction;

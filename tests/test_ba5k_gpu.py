@@ -28,7 +28,7 @@ from symforce_amd import _capi as capi
 pytestmark = pytest.mark.gpu
 
 FIXTURE = Path(__file__).resolve().parent / "golden" / "ba5k_oracle.json"
-FIXTURE_SHA256 = None  # filled in below from the committed fixture's digest
+FIXTURE_SHA256 = "b914e9de63a232c31c21688fe4f24aab9f7b6c6c5dae9f72818437f21c0b97e0"
 
 
 def problem_digest(p):
@@ -47,8 +47,7 @@ def ba5k():
 @pytest.fixture(scope="module")
 def oracle_run(ba5k):
     raw = FIXTURE.read_bytes()
-    if FIXTURE_SHA256:
-        assert hashlib.sha256(raw).hexdigest() == FIXTURE_SHA256, "fixture changed"
+    assert hashlib.sha256(raw).hexdigest() == FIXTURE_SHA256, "fixture changed"
     data = json.loads(raw)
     assert data["problem"]["sha256"] == problem_digest(ba5k), "generator output changed"
     assert data["problem"]["n_obs"] == int(ba5k.info["n_obs"])
