@@ -106,10 +106,11 @@ def run_gpu(args):
     if not args.no_converge:
         plain = time_to_cost(prob, opts, target)
         if not args.no_fast:
-            fast = time_to_cost(prob, fast_options(args), target)
+            fast = time_to_cost(prob, fast_options(args), target, early_exit=1e-7)
             alt = "f64" if args.fast_precision == "f32" else "f32"
             fast_alt = dict(precision=alt,
-                            **time_to_cost(prob, fast_options(args, precision=alt), target))
+                            **time_to_cost(prob, fast_options(args, precision=alt), target,
+                                           early_exit=1e-7))
     # the fast form's iteration rate and phases over the same K steps
     fast_ms, fast_phases, fast_lin = None, {}, None
     if not args.no_fast:
@@ -151,10 +152,15 @@ def fast_options(args, switch=True, precision=None):
                                pcg_forcing=args.pcg_forcing)
 
 
-def time_to_cost(prob, opts, target):
+def time_to_cost(prob, opts, target, early_exit=1e-6):
     """Runs one optimizer form from the initial Values to convergence and reports the wall time
-    at which its best cost first came within 1e-6 relative of the oracle's final cost."""
-    conv = capi.HipOptimizer(prob, lm_params(100), opts)
+    at which its best cost first came within 1e-6 relative of the oracle's final cost. The plain
+    form stops as the oracle's run did (relative reduction below 1e-6); the fast forms run to
+    1e-7: their exact tail converges linearly here (Gauss-Newton on a problem with nonzero
+    residuals, ~0.65 of the gap per step), so an exit at 1e-6 can stop them anywhere up to ~2e-6
+    above the minimum, on either side of the 1e-6 target from run to run. The time reported is
+    when the target is first reached, not when the run stops."""
+    conv = capi.HipOptimizer(prob, lm_params(100, early_exit=early_exit), opts)
     t0 = time.perf_counter()
     cres = conv.optimize_resident()
     t_conv = time.perf_counter() - t0
