@@ -74,7 +74,6 @@ class SupernodalLDLT {
   DBuf<uint8_t> trsm_tasks_, upd_tasks_, solve_tasks_;  // per-level task lists (chol_numeric.hip)
   // work lists (flattened over levels)
   std::vector<int32_t> lvl_snode_ptr_, lvl_trsm_ptr_, lvl_upd_ptr_, lvl_solve_ptr_;
-  std::vector<uint8_t> lvl_upd32_;  // the level's update tasks are 32-row tiles
   DBuf<int32_t> lvl_snodes_;
 };
 

@@ -18,8 +18,12 @@ namespace symhip {
 namespace {
 
 constexpr int kMaxCols = 64;  // supernode width cap (panels padded to this width)
-constexpr int kTile = 64;     // update tile (rows)
-constexpr int kSmallLevelTiles = 256;  // below this many 64-row tiles a level uses 32-row tiles
+// Update tile (rows). 32-row tiles on every level, with shared memory sized for them (33 KB, four
+// blocks per CU): a level's update time is the latency of its slowest tiles, and a 64-row tile's
+// product (K = 64 on the FP64 matrix cores) and 4096-entry scatter take ~15 us where four 32-row
+// tiles run side by side in ~6 (measured: 1000-camera system 464 -> 388 us of updates per
+// factorization, 100 000-node pose graph 460 -> 205 us).
+constexpr int kTile = 32;
 
 // l11_off: the factored diagonal block L11 (n x n, column-major, strictly lower part used) is
 // kept apart from the panel, so the row-chunk blocks of one supernode, which all read the
@@ -78,7 +82,6 @@ constexpr int kDiagRows = 2 * kMaxCols;    // diagonal block + one chunk of belo
 constexpr int kCbLd = kDiagRows + 4;       // cb[u][row]: published columns
 constexpr int kLcLd = kDiagRows + 2;       // Lc[col][row]: the working matrix, then L and D
 constexpr int kDiagSmem = kStep * kCbLd + kStep * kDiagRows + kMaxCols * kLcLd;
-constexpr int kUpdSmem = 2 * kMaxCols * (kTile + 1);
 
 // 1 / d by the hardware estimate and two Newton steps (the IEEE division sequence is twice as
 // long, and these sit on the sweep's critical path).
@@ -303,10 +306,8 @@ __global__ void __launch_bounds__(kDiagThreads) diag_trsm_kernel(
 // Update task (s, ti, tj), one 256-thread block: U = L21 D L21^T tile (ti, tj) of KT x KT,
 // scattered as -U into ancestor panels. For below-row j of the supernode, scat_col[j] is the
 // panel offset of its column in the target supernode and scat_rel[j] + i indexes upd_rel for the
-// target row position of below-row i >= j (i < mb). Levels with fewer 64-row tiles than CUs use
-// 32-row tiles: a block's product (K = 64 on the FP64 matrix cores, 4096 cycles for a 64-row
-// tile on one CU) is the latency of such a level, and a quarter of it spreads over four times
-// the blocks.
+// target row position of below-row i >= j (i < mb). The launches use KT = kTile = 32 (the body
+// also takes 64-row tiles).
 //
 // T = float is the mixed-precision form of the update: both row tiles are rounded to FP32 (the
 // D scaling applied first, in FP64), the product runs on the FP32 matrix cores
@@ -417,7 +418,7 @@ __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ 
                                                      const int64_t* __restrict__ scat_col,
                                                      const int32_t* __restrict__ scat_rel,
                                                      const int32_t* __restrict__ upd_rel) {
-  __shared__ double smem[kUpdSmem];
+  __shared__ double smem[2 * kMaxCols * (KT + 1)];
   update_body<KT, T>(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
@@ -646,9 +647,7 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
   lvl_solve_ptr_.assign(1, 0);
   lvl_trsm_ptr_.assign(1, 0);
   lvl_upd_ptr_.assign(1, 0);
-  lvl_upd32_.assign(sym.n_levels, 0);
   for (int l = 0; l < sym.n_levels; ++l) {
-    int64_t tiles64 = 0;
     for (int q = sym.level_ptr[l]; q < sym.level_ptr[l + 1]; ++q) {
       const int sn = sym.level_snodes[q];
       snl.push_back(sn);
@@ -664,12 +663,8 @@ void SupernodalLDLT::setup(const CholSymbolic& sym, const int32_t* colptr, const
         sol.push_back(sn);
         sol.push_back(c);
       }
-      const int T = (mb + kTile - 1) / kTile;
-      tiles64 += T * (T + 1) / 2;
     }
-    // update tiles: 32 rows where the 64-row tiles would not fill the CUs
-    const int kt = tiles64 < kSmallLevelTiles ? kTile / 2 : kTile;
-    lvl_upd32_[l] = kt != kTile;
+    constexpr int kt = kTile;
     for (int q = sym.level_ptr[l]; q < sym.level_ptr[l + 1]; ++q) {
       const int sn = sym.level_snodes[q];
       const int n = sym.sn_col_begin[sn + 1] - sym.sn_col_begin[sn];
@@ -785,9 +780,7 @@ void SupernodalLDLT::launch_factorize(const double* b, hipStream_t s) {
     }
     const int nu = lvl_upd_ptr_[l + 1] - lvl_upd_ptr_[l];
     if (nu) {
-      const bool t32 = lvl_upd32_[l];
-      auto kern = f32_updates_ ? (t32 ? update_kernel<kTile / 2, float> : update_kernel<kTile, float>)
-                               : (t32 ? update_kernel<kTile / 2, double> : update_kernel<kTile, double>);
+      auto kern = f32_updates_ ? update_kernel<kTile, float> : update_kernel<kTile, double>;
       hipLaunchKernelGGL(kern, dim3(nu), dim3(256), 0, s, upd + lvl_upd_ptr_[l], panel_.get(),
                          D_.get(), scat_col_.get(), scat_rel_.get(), upd_rel_.get());
     }

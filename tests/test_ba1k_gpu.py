@@ -5,7 +5,8 @@ observations; the benchmark's workload).
   this problem (79 records, 45 minutes on one core; tests/golden/make_ba1k_oracle.py). The plain
   FP64 device run to convergence must follow its first iterations to 1e-8 relative in new_error
   with the same accept decisions and land on its final cost to 1e-9; the fast form (FP32
-  storage, block-Jacobi PCG, exact LDL^T late) must reach it to 1e-6. The oracle orders the
+  storage, block-Jacobi PCG, exact LDL^T late) must reach it (its cost at most 1e-6 above the
+  oracle's final cost; run to a 1e-7 reduction it goes below it). The oracle orders the
   reduced camera system naturally and the device by nested dissection, so the two runs differ
   only by rounding in the factorization.
 * A size-independent property: the exact Schur step (reduced camera system assembled on the
@@ -88,13 +89,17 @@ def test_ba1k_fast_form_reaches_the_oracle_cost(ba1k, oracle_run, precision):
                                use_pcg=1, pcg_max_iters=20, pcg_rel_tol=3e-2,
                                pcg_exact_below_reduction=1e-4)
     # run to a relative reduction of 1e-7: after the switch to exact steps the fast form converges
-    # linearly (~0.65 of the gap per step), so the 1e-6 exit can stop it up to ~2e-6 above the
-    # minimum; the bar is reaching the oracle's cost
+    # linearly, so the 1e-6 exit can stop it up to ~2e-6 above the oracle's cost. The bar is
+    # reaching the oracle's cost (one-sided): the oracle's own run ended at its 1e-6 exit, and run
+    # further the form goes below it (the valley keeps falling at ~2e-7 per iteration)
     params = capi.default_params(lambda_update_type=capi.LAMBDA_DYNAMIC, iterations=100,
                                  early_exit_min_reduction=1e-7)
     hip = capi.HipOptimizer(ba1k, params, opts).optimize()
-    assert hip["status"] == capi.STATUS_SUCCESS
-    assert _best(hip) == pytest.approx(oracle_run["final_cost"], rel=1e-6)
+    assert hip["status"] in (capi.STATUS_SUCCESS, capi.STATUS_HIT_ITERATION_LIMIT)
+    its = hip["iterations"]
+    print(f"\nfast form ({precision}): {len(its) - 1} iterations, best {_best(hip)!r}, last "
+          f"relative reductions {[round(it['relative_reduction'], 9) for it in its[-6:]]}")
+    assert _best(hip) <= oracle_run["final_cost"] * (1 + 1e-6)
 
 
 def test_ba1k_exact_step_matches_tight_pcg(ba1k):
