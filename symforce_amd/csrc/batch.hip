@@ -210,12 +210,14 @@ __device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36],
       }
     }
     // Cholesky of M; every entry of C_i ends up in Cb (uniform) on the way
-    double Cb[6][6];
+    // (one reciprocal per column; the triangular solves below multiply by it)
+    double Cb[6][6], Ci[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double ckk = sqrt(lane_bcast(m[k], k));
       Cb[k][k] = ckk;
-      const double lrk = m[k] / ckk;  // row r's entry of column k (r > k)
+      Ci[k] = 1.0 / ckk;
+      const double lrk = m[k] * Ci[k];  // row r's entry of column k (r > k)
 #pragma unroll
       for (int c = k + 1; c < 6; ++c) Cb[c][k] = lane_bcast(lrk, c);
       if (r > k) m[k] = lrk;
@@ -237,7 +239,7 @@ __device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36],
         double v = B[i][r * 6 + c];
 #pragma unroll
         for (int k = 0; k < c; ++k) v -= w[k] * Cb[c][k];
-        w[c] = v / Cb[c][c];
+        w[c] = v * Ci[c];
       }
     }
     double b = x[i][r];
@@ -253,7 +255,7 @@ __device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36],
       double v = bu[q];
 #pragma unroll
       for (int k = 0; k < q; ++k) v -= Cb[q][k] * y[k];
-      y[q] = v / Cb[q][q];
+      y[q] = v * Ci[q];
     }
     if (row) {
       x[i][r] = y[r < 6 ? r : 5];
@@ -277,15 +279,18 @@ __device__ void block_tridiagonal_solve(int n, double (*A)[21], double (*B)[36],
 #pragma unroll
       for (int k = 0; k < 6; ++k) b -= B[i][k * 6 + r] * xnext[k];
     }
-    double bu[6], xs[6];
+    double bu[6], xs[6], di[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) bu[k] = lane_bcast(b, k);
+    for (int k = 0; k < 6; ++k) {
+      bu[k] = lane_bcast(b, k);
+      di[k] = 1.0 / A[i][tri(k, k)];  // off the substitution's dependent chain
+    }
 #pragma unroll
     for (int q = 5; q >= 0; --q) {
       double v = bu[q];
 #pragma unroll
       for (int k = q + 1; k < 6; ++k) v -= A[i][tri(k, q)] * xs[k];
-      xs[q] = v / A[i][tri(q, q)];
+      xs[q] = v * di[q];
     }
     if (row) x[i][r] = xs[r < 6 ? r : 5];
 #pragma unroll
