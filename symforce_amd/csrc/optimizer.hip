@@ -465,7 +465,6 @@ class CscBackend final : public SystemBackend {
   int32_t M() const override { return M_; }
   int64_t nnz() const override { return nnz_; }
   SupernodalLDLT& chol() { return chol_; }
-  bool mixed() const { return chol_.f32_updates(); }
   void info(int64_t* out) const override {
     const CholSymbolic& y = chol_.symbolic();
     out[0] = y.n;
@@ -521,10 +520,7 @@ class CscBackend final : public SystemBackend {
         hipLaunchKernelGGL(cg_step_kernel, g, blk, 0, s, N_, rz, sc + 2, p, q, x, r);
         dot(r, r, N_, partials_.get(), sc + 1, s);
         rel = read();
-        if (!(host_sums_[2] > 0) || !std::isfinite(rel)) {  // breakdown
-          ++breakdowns_;
-          break;
-        }
+        if (!(host_sums_[2] > 0) || !std::isfinite(rel)) break;  // breakdown
         // converged, or stagnating at the FP64 floor of this system (no halving of the best
         // residual in 3 iterations; accepted below kFallbackTol)
         since_best = rel < 0.5 * best ? 0 : since_best + 1;
@@ -604,7 +600,6 @@ class CscBackend final : public SystemBackend {
   DBuf<int32_t> rowind_, tptr_, tcol_;
   DBuf<int64_t> tidx_;
   int64_t solves_ = 0, refine_steps_ = 0, max_refine_steps_ = 0, fallbacks_ = 0;
-  int64_t breakdowns_ = 0;  // CG breakdowns among the fallbacks
   bool track_ = false;
   std::vector<double> hist_;
   DBuf<double> residual_[3], hval_[3], rhs_[3];
