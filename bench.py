@@ -19,7 +19,8 @@ Prints one JSON line (value = LM iterations/s of the timed FP64 run) with:
                 analysis excluded), the reduced camera system ordered by nested dissection as the
                 reference orders it with METIS; also the -ffast-math build of the same sources
   batch         4096 robot-3D-localization problems in one persistent launch (FP64 and FP32
-                Jacobian forms), against the CPU oracle on one core and on all cores
+                Jacobian forms; launch autotuner; the kernel's FP64 issue rate from the committed
+                PMC summary), against the CPU oracle on one core and on all cores
   pose_graph    the 100 000-node pose graph (Cholesky path): LM iterations/s, phases, the
                 mixed-precision factorization form, run to convergence, the factorization
                 phase's roofline, and the CPU oracle on the same problem and elimination order
@@ -247,6 +248,19 @@ def host_cpu():
     return model, len(os.sched_getaffinity(0))
 
 
+def batch_flops_table():
+    """Issued FP64 lane-flops per launch of each batched-kernel variant, from the committed PMC
+    summary (profiles/batch_localization_f64_ops.txt)."""
+    path = REPO / "profiles" / "batch_localization_f64_ops.txt"
+    rows = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            if line.startswith("batch_lm_kernel"):
+                name, rest = line[:30].strip(), line[30:].split()
+                rows[name] = float(rest[-1])
+    return rows
+
+
 def batch_leg(n_problems=4096, cpu_sample=64):
     """Batched form: n independent robot-3D-localization problems (5-50 poses, 5-40 landmarks,
     30 % visible) in one persistent launch, FP64 and FP32-Jacobian forms; the CPU oracle solves
@@ -261,6 +275,9 @@ def batch_leg(n_problems=4096, cpu_sample=64):
         runs = [b.solve(records=False) for _ in range(3)]
         r = min(runs, key=lambda x: x["total_ms"])
         ok = sum(p["status"] == capi.STATUS_SUCCESS for p in r["problems"])
+        kernel = (f"batch_lm_kernel<{'double' if prec == capi.PRECISION_F64 else 'float'}, "
+                  f"{2 if chosen & 1 else 1}>")
+        flops = batch_flops_table().get(kernel)
         out[name] = {"problems_per_s": n_problems / (r["total_ms"] * 1e-3),
                      "launch_ms": r["total_ms"],
                      "p50_solve_ms": float(np.percentile(r["solve_ms"], 50)),
@@ -268,7 +285,15 @@ def batch_leg(n_problems=4096, cpu_sample=64):
                      "converged": ok,
                      "autotune": {"chosen": capi.HipBatch.BATCH_CONFIGS[chosen],
                                   "config_ms": {capi.HipBatch.BATCH_CONFIGS[c]: float(t)
-                                                for c, t in enumerate(config_ms)}}}
+                                                for c, t in enumerate(config_ms)}},
+                     # FP64 VALU issue (masked lanes included) against the FP64 vector peak:
+                     # the kernel is latency-bound (per-problem dependent chains)
+                     "roofline": None if flops is None else {
+                         "kernel": kernel, "bound": "latency",
+                         "achieved": flops / (r["total_ms"] * 1e-3) / 1e12,
+                         "peak": 78.6, "unit": "TFLOP/s (FP64 vector)",
+                         "frac": flops / (r["total_ms"] * 1e-3) / 1e12 / 78.6,
+                         "issued_fp64_lane_flops_per_launch": flops}}
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_wrap as ow
     from concurrent.futures import ThreadPoolExecutor
