@@ -189,7 +189,7 @@ def roofline(r):
     """The dominant work of an LM iteration is the supernodal LDL^T factorization phase: per
     level of the supernodal elimination tree one diagonal kernel (diag_trsm_kernel: each
     supernode's diagonal block and its panel rows eliminated in one right-looking sweep, the
-    forward solve fused in) and one update kernel (update_kernel<64|32>: the trailing products
+    forward solve fused in) and one update kernel (update_kernel<32>: the trailing products
     into ancestors on the FP64 matrix cores). Its algorithmic work is the symbolic analysis'
     flop count (per supernode cols^3/3 + below x cols^2 + below^2 x cols), and the phase timer
     brackets the phase with HIP events on the optimizer's stream, so achieved = flops / phase
@@ -218,7 +218,7 @@ def roofline(r):
     rcs_traffic = table.get("void rcs_assemble_kernel<9, double, double>", (0, None))[1]
     rcs_traffic = None if rcs_traffic is None else rcs_traffic * 1e6
     return dict(
-        kernel=(f"supernodal LDL^T factorization phase: diag_trsm_kernel + update_kernel<64|32>"
+        kernel=(f"supernodal LDL^T factorization phase: diag_trsm_kernel + update_kernel<32>"
                 f" per level, {info['levels']} levels"),
         bound="mfma", achieved=fac_tf, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
         frac=fac_tf / FP64_PEAK_TFLOPS, traffic=fac_traffic, ms_per_launch=fac_ms,
