@@ -44,11 +44,14 @@ constexpr int z_stride() {
 // (ND x 3 row-major, stored as TW), and add its J_point^T J_point (lower 6: 00 10 11 20 21 22) and
 // J_point^T r to the point's V_p and g_p. Observations are point-major, so a point's terms sit in
 // consecutive lanes: a segmented shuffle scan sums them inside the wavefront and the last lane of
-// each segment adds the sum with one atomic per value (V and g start at zero).
+// each segment adds the sum with one atomic per value (V and g start at zero). A wavefront's W
+// rows go through LDS and out as 16-byte pieces from consecutive lanes (a lane storing its own
+// 216-byte row writes 64 partial cache lines per instruction: 119 -> 101 us per launch at 1000
+// cameras, in blocks of two wavefronts so that the staging leaves room for five blocks per CU).
 // TW = float is the FP32 form: the Hessian-side products (W, V) come from the FP32 factor body;
 // the residual and the gradient J_point^T r stay FP64.
 template <int FTC, int ND, typename TW>
-__global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
+__global__ void __launch_bounds__(128) lin_points_kernel(int32_t n_obs,
                                                          const int32_t* __restrict__ obs_point,
                                                          ObsView ov, const double* __restrict__ values,
                                                          double* __restrict__ residual,
@@ -57,8 +60,11 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
                                                          double* __restrict__ gp) {
   using F = FT<FTC>;
   static_assert(F::T == ND + 3, "landmark factor layout");
+  constexpr int kW = ND * 3;
+  __shared__ TW wst[2][64 * kW];  // each wavefront's W rows, copied out coalesced at the end
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  TW* ws = wst[threadIdx.x >> 6];
   const bool live = o < n_obs;
   double acc[9] = {};  // v[6], g[3]
   int p = -1 - lane;   // distinct per dead lane so dead lanes never join a segment
@@ -72,7 +78,7 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
     const int ro = ov.res_off[o];
 #pragma unroll
     for (int i = 0; i < F::D; ++i) residual[ro + i] = r[i];
-    TW* Wo = W + (int64_t)o * ND * 3;
+    TW* Wo = ws + lane * kW;
     // Hessian-side products in the storage precision
     auto hess = [&](const auto (&Jh)[F::D][F::T]) {
       using S = std::remove_cv_t<std::remove_reference_t<decltype(Jh[0][0])>>;
@@ -109,6 +115,25 @@ __global__ void __launch_bounds__(256) lin_points_kernel(int32_t n_obs,
 #pragma unroll
       for (int q = 0; q < F::D; ++q) gi += J[q][ND + i] * r[q];
       acc[6 + i] = gi;
+    }
+  }
+  // the wavefront's W rows are one contiguous range: 16-byte pieces from consecutive lanes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  {
+    const int o0 = o - lane;
+    const int cnt = min(64, n_obs - o0);
+    TW* dst = W + (int64_t)o0 * kW;
+    constexpr int kVec = 16 / (int)sizeof(TW);
+    typedef TW v16 __attribute__((ext_vector_type(kVec)));
+    static_assert((64 * kW) % kVec == 0, "a full wavefront's rows are whole pieces");
+    if (cnt == 64) {
+#pragma unroll 2
+      for (int m = lane; m < 64 * kW / kVec; m += 64)
+        *reinterpret_cast<v16*>(dst + m * kVec) = *reinterpret_cast<const v16*>(ws + m * kVec);
+    } else {
+      for (int e = lane; e < cnt * kW; e += 64) dst[e] = ws[e];
     }
   }
   // inclusive segmented scan over lanes with equal p (segments are contiguous)
@@ -581,8 +606,13 @@ __global__ void __launch_bounds__(256) rcs_assemble_kernel(
   }
 }
 
-// Per camera node (wavefront): v_a = g_a - sum_{o in a} Z_o h_p(o) and the diagonal-block
-// elimination term ZZ_a = sum_{o in a} Z_o Z_o^T (full ND x ND, row-major).
+// Per camera node (a block of four wavefronts): v_a = g_a - sum_{o in a} Z_o h_p(o) and the
+// diagonal-block elimination term ZZ_a = sum_{o in a} Z_o Z_o^T (full ND x ND, row-major). The
+// wavefronts take every fourth chunk of 64 of the camera's observations; a chunk's Z rows are
+// copied into LDS with coalesced 16-byte loads (a few consecutive lanes per row), each lane sums
+// its observation's terms from its LDS row, and the four wavefronts' sums are added at the end.
+// (One wavefront per camera, each lane loading its own row: 90 us per launch at 1000 cameras;
+// this form 72 us: the chain of passes over a camera's ~1000 observations is a quarter as long.)
 template <int ND, typename TW>
 __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           const int32_t* __restrict__ cam_obs_ptr,
@@ -594,54 +624,87 @@ __global__ void __launch_bounds__(256) reduced_rhs_kernel(int32_t n_nodes,
                                                           double* __restrict__ v,
                                                           double* __restrict__ ZZ) {
   constexpr int NH = ND * (ND + 1) / 2;
-  __shared__ double scr[4][16][65];  // per-wavefront reduction staging (no block barrier)
-  const int wave = xcd_tile(blockIdx.x, (n_nodes + 3) / 4) * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (wave >= n_nodes) return;
+  constexpr int ZS = z_stride<ND, TW>();
+  constexpr int ZL = ZS + 1;                  // LDS row stride (odd: conflict-free row reads)
+  constexpr int kVec = 16 / (int)sizeof(TW);  // Z entries per 16-byte piece
+  constexpr int kPpo = ZS / kVec;             // pieces per observation
+  constexpr int kZl = 64 * ZL;                // one wavefront's staged rows
+  static_assert(kZl >= 16 * 65, "the reduction staging fits in the row staging");
+  __shared__ double lds[4][kZl];
+  __shared__ double red[4][ND + NH];
+  const int cam = xcd_tile(blockIdx.x, n_nodes);
+  if (cam >= n_nodes) return;  // block-uniform
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* zs = lds[wave];
   double acc[ND] = {}, zz[NH] = {};
-  for (int q = cam_obs_ptr[wave] + lane; q < cam_obs_ptr[wave + 1]; q += 64) {
-    const int o = cam_obs[q];
-    const int p = obs_point[o];
-    const double h0 = hp[p * 3], h1 = hp[p * 3 + 1], h2 = hp[p * 3 + 2];
-    const TW* Zo = Z + (int64_t)o * z_stride<ND, TW>();
-    double z[ND * 3];
+  const int q_end = cam_obs_ptr[cam + 1];
+  for (int q0 = cam_obs_ptr[cam] + 64 * wave; q0 < q_end; q0 += 256) {
+    // the 64 observations' Z rows, each 16-byte piece by a lane (rows clamped to the range)
+    typedef TW v16 __attribute__((ext_vector_type(kVec)));
 #pragma unroll
-    for (int e = 0; e < ND * 3; ++e) z[e] = (double)Zo[e];
+    for (int m = 0; m < kPpo; ++m) {
+      const int piece = lane + 64 * m;
+      const int ol = piece / kPpo, within = (piece % kPpo) * kVec;
+      const int o = cam_obs[min(q0 + ol, q_end - 1)];
+      const v16 zv = *reinterpret_cast<const v16*>(Z + (int64_t)o * ZS + within);
 #pragma unroll
-    for (int i = 0; i < ND; ++i) acc[i] += z[i * 3] * h0 + z[i * 3 + 1] * h1 + z[i * 3 + 2] * h2;
-    int idx = 0;
+      for (int e = 0; e < kVec; ++e) zs[ol * ZL + within + e] = (double)zv[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (q0 + lane < q_end) {
+      const int p = obs_point[cam_obs[q0 + lane]];
+      const double h0 = hp[p * 3], h1 = hp[p * 3 + 1], h2 = hp[p * 3 + 2];
+      double z[ND * 3];
 #pragma unroll
-    for (int i = 0; i < ND; ++i)
+      for (int e = 0; e < ND * 3; ++e) z[e] = zs[lane * ZL + e];
 #pragma unroll
-      for (int k = 0; k <= i; ++k)
-        zz[idx++] += z[i * 3] * z[k * 3] + z[i * 3 + 1] * z[k * 3 + 1] + z[i * 3 + 2] * z[k * 3 + 2];
+      for (int i = 0; i < ND; ++i) acc[i] += z[i * 3] * h0 + z[i * 3 + 1] * h1 + z[i * 3 + 2] * h2;
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+#pragma unroll
+        for (int k = 0; k <= i; ++k)
+          zz[idx++] += z[i * 3] * z[k * 3] + z[i * 3 + 1] * z[k * 3 + 1] + z[i * 3 + 2] * z[k * 3 + 2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next pass (or the reduction) overwrites the rows
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   double az[ND + NH];
 #pragma unroll
   for (int i = 0; i < ND; ++i) az[i] = acc[i];
 #pragma unroll
   for (int i = 0; i < NH; ++i) az[ND + i] = zz[i];
-  double* zo = ZZ + (int64_t)wave * ND * ND;
-  wave_sum_lds<ND + NH>(az, scr[threadIdx.x >> 6], [&](int e, double t) {
+  wave_sum_lds<ND + NH>(az, reinterpret_cast<double (*)[65]>(zs),
+                        [&](int e, double t) { red[wave][e] = t; });
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e < ND + NH) {
+    const double t = red[0][e] + red[1][e] + red[2][e] + red[3][e];
     if (e < ND) {
-      v[wave * ND + e] = gB[wave * ND + e] - t;
+      v[cam * ND + e] = gB[cam * ND + e] - t;
     } else {
       const int p = e - ND;  // packed lower entry (i, k), k <= i
       int i = 0;
       while ((i + 1) * (i + 2) / 2 <= p) ++i;
       const int k = p - i * (i + 1) / 2;
+      double* zo = ZZ + (int64_t)cam * ND * ND;
       zo[i * ND + k] = t;
       zo[k * ND + i] = t;
     }
-  });
+  }
 }
 
 // Per point: z_p = L_p^-T (h_p - sum_o Z_o^T y_node(o)); write -z (and -y for the nodes) into the
 // update vector in reference state order. A wavefront takes 64 consecutive points, whose
-// observations are one contiguous range: lanes take consecutive observations (coalesced Z rows
-// instead of one thread walking each point's rows), stage Z_o^T y_node(o) in LDS a chunk at a
-// time, and each lane then sums the observations of its own point.
-constexpr int kBackSubChunk = 128;  // observations staged per wavefront and pass
+// observations are one contiguous range of Z rows: a chunk of 64 of them is copied into LDS with
+// coalesced 16-byte loads (consecutive lanes, consecutive pieces), each lane forms one
+// observation's Z_o^T y_node(o) from its LDS row, and each lane then sums the observations of its
+// own point. (A lane loading its own observation's 224-byte row touches 64 cache lines per
+// instruction and re-requests each line ~8 times: 114 -> 65 us per launch at 1000 cameras.)
+constexpr int kBackSubChunk = 64;  // observations staged per wavefront and pass
 
 template <int ND, typename TW>
 __global__ void __launch_bounds__(256) back_substitute_kernel(
@@ -649,6 +712,11 @@ __global__ void __launch_bounds__(256) back_substitute_kernel(
     const TW* __restrict__ Z, const double* __restrict__ Lp, const double* __restrict__ hp,
     const double* __restrict__ y, const int32_t* __restrict__ pt_tan, double* __restrict__ update) {
   constexpr int ZS = z_stride<ND, TW>();
+  constexpr int ZL = ZS + 1;                   // LDS row stride (odd: conflict-free row reads)
+  constexpr int kVec = 16 / (int)sizeof(TW);   // Z entries per 16-byte piece
+  constexpr int kPieces = kBackSubChunk * ZS / kVec;
+  static_assert(kPieces % 64 == 0 && ZS % kVec == 0, "whole pieces per lane and observation");
+  __shared__ double zs[4][kBackSubChunk * ZL];
   __shared__ double ts[4][kBackSubChunk][3];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int p0 = (blockIdx.x * 4 + wave) * 64;
@@ -659,22 +727,35 @@ __global__ void __launch_bounds__(256) back_substitute_kernel(
   const int my_b = pt_obs_ptr[pc], my_e = p < n_points ? pt_obs_ptr[pc + 1] : my_b;
   double t0 = 0, t1 = 0, t2 = 0;
   for (int c0 = o_begin; c0 < o_end; c0 += kBackSubChunk) {
+    // the chunk's Z rows are contiguous: coalesced 16-byte pieces into LDS (rows clamped)
+    typedef TW v16 __attribute__((ext_vector_type(kVec)));
 #pragma unroll
-    for (int q = lane; q < kBackSubChunk; q += 64) {
-      const int o = min(c0 + q, o_end - 1);  // clamped; rows past the range are never summed
-      const TW* Zo = Z + (int64_t)o * ZS;
+    for (int m = 0; m < kPieces / 64; ++m) {
+      const int piece = lane + 64 * m;
+      const int ol = piece / (ZS / kVec), within = (piece % (ZS / kVec)) * kVec;
+      const int o = min(c0 + ol, o_end - 1);
+      const v16 zv = *reinterpret_cast<const v16*>(Z + (int64_t)o * ZS + within);
+#pragma unroll
+      for (int e = 0; e < kVec; ++e) zs[wave][ol * ZL + within + e] = (double)zv[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {
+      const int o = min(c0 + lane, o_end - 1);
       const double* yn = y + obs_node[o] * ND;
+      const double* zr = &zs[wave][lane * ZL];
       double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int i = 0; i < ND; ++i) {
         const double yi = yn[i];
-        s0 += (double)Zo[i * 3] * yi;
-        s1 += (double)Zo[i * 3 + 1] * yi;
-        s2 += (double)Zo[i * 3 + 2] * yi;
+        s0 += zr[i * 3] * yi;
+        s1 += zr[i * 3 + 1] * yi;
+        s2 += zr[i * 3 + 2] * yi;
       }
-      ts[wave][q][0] = s0;
-      ts[wave][q][1] = s1;
-      ts[wave][q][2] = s2;
+      ts[wave][lane][0] = s0;
+      ts[wave][lane][1] = s1;
+      ts[wave][lane][2] = s2;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1329,8 +1410,8 @@ void SchurBackend::relin_nd(int b, const double* values, hipStream_t s) {
         w_f32_[b] = f32_ && !exact_now_;
         with_storage(b, [&](auto* W, auto*) {
           using TW = std::remove_pointer_t<decltype(W)>;
-          hipLaunchKernelGGL((lin_points_kernel<C, ND, TW>), dim3(grid_for(n_obs_, 256)),
-                             dim3(256), 0, s, n_obs_, obs_point_.get(), ov, values,
+          hipLaunchKernelGGL((lin_points_kernel<C, ND, TW>), dim3(grid_for(n_obs_, 128)),
+                             dim3(128), 0, s, n_obs_, obs_point_.get(), ov, values,
                              residual_[b].get(), W, V_[b].get(), gp_[b].get());
         });
       }
@@ -1425,7 +1506,7 @@ void SchurBackend::solve_nd(int b, double* update, hipStream_t s) {
     PhaseScope ps(timer_, SYMHIP_PHASE_REDUCED_RHS, s);
     with_storage(b, [&](auto*, auto* Z) {
       using TW = std::remove_pointer_t<decltype(Z)>;
-      hipLaunchKernelGGL((reduced_rhs_kernel<ND, TW>), dim3(xcd_grid((n_nodes_ + 3) / 4)),
+      hipLaunchKernelGGL((reduced_rhs_kernel<ND, TW>), dim3(xcd_grid(n_nodes_)),
                          dim3(256), 0, s, n_nodes_, cam_obs_ptr_.get(), cam_obs_.get(),
                          obs_point_.get(), Z, hp_.get(), gB_[b].get(), v_.get(), ZZ_.get());
     });
