@@ -24,6 +24,7 @@ constexpr int kMaxCols = 64;  // supernode width cap (panels padded to this widt
 // tiles run side by side in ~6 (measured: 1000-camera system 464 -> 388 us of updates per
 // factorization, 100 000-node pose graph 460 -> 205 us).
 constexpr int kTile = 32;
+constexpr int kCus = 256;  // compute units of an MI355X (block-count thresholds of the launches)
 
 // l11_off: the factored diagonal block L11 (n x n, column-major, strictly lower part used) is
 // kept apart from the panel, so the row-chunk blocks of one supernode, which all read the
@@ -509,6 +510,13 @@ __global__ void __launch_bounds__(256) fwd_kernel(const SnTask* __restrict__ tas
 // counter, reset by that block for the next solve) then forms y_own = L11^-T (D^-1 y_own - part)
 // with one wavefront (the D^-1 of the solve folded in: own entries are read by their supernode's
 // blocks only). part must be zero on entry (the permutation kernel clears it).
+//
+// kLean (levels with more blocks than fit two to a CU, e.g. the thousands of small leaf
+// supernodes of a pose graph): L11 is held in registers instead of LDS until the block knows it
+// is the last, then written over the partials' space, so the block needs 33 KB of LDS instead of
+// 66 KB and four blocks fit a CU instead of two (the 100 000-node pose graph's backward solve
+// 0.43 -> 0.35 ms). On levels of a few blocks the extra barrier only adds latency.
+template <bool kLean>
 __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ rows,
                                                   const double* __restrict__ panel,
@@ -517,8 +525,13 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
                                                   double* __restrict__ yout,
                                                   double* __restrict__ part,
                                                   int32_t* __restrict__ counter) {
-  __shared__ double Ls[kMaxCols][kMaxCols + 1];
-  __shared__ double red_all[4][(kMaxCols / 4) * (64 + 1)];  // per-wave column partials
+  // the waves' column partials and L11; with kLean, L11 takes the partials' space at the end
+  constexpr int kRed = (kMaxCols / 4) * (64 + 1);
+  constexpr int kLs = kMaxCols * (kMaxCols + 1);
+  static_assert(4 * kRed >= kLs, "L11 fits the partials' space");
+  __shared__ double buf[4 * kRed + (kLean ? 0 : kLs)];
+  auto red_all = reinterpret_cast<double (*)[kRed]>(buf);
+  auto Ls = reinterpret_cast<double (*)[kMaxCols + 1]>(kLean ? buf : buf + 4 * kRed);
   __shared__ int last;
   const SnTask tk = tasks[blockIdx.x];
   const int s = tk.s, chunk = tk.a;
@@ -528,10 +541,23 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
   const double* P = panel + sd.panel_off + n;
   const int32_t* rw = rows + sd.rows_off + n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // Every block stages L11 and reads y_own up front: only the last block of s uses them, but this
-  // way their loads overlap the partial sums instead of following the arrival counter (y_own is
+  // Every block loads L11 and y_own up front: only the last block of s uses them, but this way
+  // their loads overlap the partial sums instead of following the arrival counter (y_own is
   // written by no other block of this launch).
-  stage_l11(l11 + sd.l11_off, n, Ls);
+  constexpr int kL11 = kLean ? kMaxCols * kMaxCols / 256 : 1;  // blockDim.x == 256
+  double l11v[kL11];
+  if constexpr (kLean) {
+    const double* Lc = l11 + sd.l11_off;
+#pragma unroll
+    for (int t = 0; t < kL11; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      const int i = idx % kMaxCols, k = idx / kMaxCols;
+      const double v = Lc[(int64_t)min(k, n - 1) * n + min(i, n - 1)];  // clamped, then masked
+      l11v[t] = (i < n && k < n && i > k) ? v : 0.0;
+    }
+  } else {
+    stage_l11(l11 + sd.l11_off, n, Ls);
+  }
   double yown = 0;
   if (threadIdx.x < n) yown = yout[sd.col_begin + threadIdx.x] / D[sd.col_begin + threadIdx.x];
   if (mb > 0) {
@@ -577,6 +603,14 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
            nchunks - 1;
   __syncthreads();
   if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
+  if constexpr (kLean) {  // every wave is past its partials (the barriers above)
+#pragma unroll
+    for (int t = 0; t < kL11; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      Ls[idx % kMaxCols][idx / kMaxCols] = l11v[t];
+    }
+    __syncthreads();
+  }
   // every chunk's sums are in part: read them at agent scope (past the per-XCD L2)
   if (threadIdx.x < 64) {
     const int r = threadIdx.x;
@@ -809,8 +843,8 @@ void SupernodalLDLT::launch_backward(double* x, hipStream_t s) {
   const SnTask* tasks = reinterpret_cast<const SnTask*>(solve_tasks_.get());
   for (int l = sym_.n_levels - 1; l >= 0; --l) {
     const int nb = lvl_solve_ptr_[l + 1] - lvl_solve_ptr_[l];
-    hipLaunchKernelGGL(bwd_kernel, dim3(nb), dim3(256), 0, s,
-                       tasks + lvl_solve_ptr_[l], rows_.get(),
+    auto kern = nb > 2 * kCus ? bwd_kernel<true> : bwd_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, tasks + lvl_solve_ptr_[l], rows_.get(),
                        panel_.get(), l11_.get(), D_.get(), yout_.get(), part_.get(),
                        counter_.get());
   }
