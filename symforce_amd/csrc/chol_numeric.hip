@@ -509,7 +509,9 @@ __global__ void __launch_bounds__(256) fwd_kernel(const SnTask* __restrict__ tas
 // wave), added atomically into part; the last chunk block of s to finish (a per-supernode arrival
 // counter, reset by that block for the next solve) then forms y_own = L11^-T (D^-1 y_own - part)
 // with one wavefront (the D^-1 of the solve folded in: own entries are read by their supernode's
-// blocks only). part must be zero on entry (the permutation kernel clears it).
+// blocks only). part must be zero on entry (the permutation kernel clears it). A supernode of
+// one chunk (at most kSolveRows below-rows: most of a pose graph's) skips part, the atomics and
+// the arrival: its block's column sums are final and go through LDS.
 //
 // kLean (levels with more blocks than fit two to a CU, e.g. the thousands of small leaf
 // supernodes of a pose graph): L11 is held in registers instead of LDS until the block knows it
@@ -532,6 +534,7 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
   __shared__ double buf[4 * kRed + (kLean ? 0 : kLs)];
   auto red_all = reinterpret_cast<double (*)[kRed]>(buf);
   auto Ls = reinterpret_cast<double (*)[kMaxCols + 1]>(kLean ? buf : buf + 4 * kRed);
+  __shared__ double colsum[kMaxCols];  // a single-chunk supernode's column sums
   __shared__ int last;
   const SnTask tk = tasks[blockIdx.x];
   const int s = tk.s, chunk = tk.a;
@@ -591,18 +594,26 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
     v += __shfl_xor(v, 1, 64);
     v += __shfl_xor(v, 2, 64);
     const int k = wave + 4 * c;
-    if (qtr == 0 && k < n) unsafeAtomicAdd(&part[sd.col_begin + k], v);
+    if (qtr == 0 && k < n) {
+      if (nchunks == 1) colsum[k] = v;  // the block holds every row: the sums are final
+      else unsafeAtomicAdd(&part[sd.col_begin + k], v);
+    }
   }
-  // Every wave waits for its own atomics into part before the block arrives: a workgroup
-  // barrier does not wait for the memory operations of the other waves, and an agent fence in
-  // the arriving thread would cover only its own wave (and write back and invalidate the L2).
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0)
-    last = __hip_atomic_fetch_add(&counter[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-           nchunks - 1;
-  __syncthreads();
-  if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
+  if (nchunks == 1) {  // uniform: no other block of s, no arrival
+    __syncthreads();
+  } else {
+    // Every wave waits for its own atomics into part before the block arrives: a workgroup
+    // barrier does not wait for the memory operations of the other waves, and an agent fence
+    // in the arriving thread would cover only its own wave (and write back and invalidate the
+    // L2).
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      last = __hip_atomic_fetch_add(&counter[s], 1, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == nchunks - 1;
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(last)) return;  // uniform across the block
+  }
   if constexpr (kLean) {  // every wave is past its partials (the barriers above)
 #pragma unroll
     for (int t = 0; t < kL11; ++t) {
@@ -615,13 +626,17 @@ __global__ void __launch_bounds__(256) bwd_kernel(const SnTask* __restrict__ tas
   if (threadIdx.x < 64) {
     const int r = threadIdx.x;
     double yi = 0;
-    if (r < n)
-      yi = yown - __hip_atomic_load(&part[sd.col_begin + r], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+    if (r < n) {
+      if (nchunks == 1)
+        yi = mb > 0 ? yown - colsum[r] : yown;
+      else
+        yi = yown - __hip_atomic_load(&part[sd.col_begin + r], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    }
 #pragma unroll
     for (int k = kMaxCols - 1; k >= 0; --k) yi -= Ls[k][r] * lane_value(yi, k);
     if (r < n) yout[sd.col_begin + r] = yi;
-    if (r == 0) counter[s] = 0;
+    if (r == 0 && nchunks > 1) counter[s] = 0;
   }
 }
 
