@@ -18,8 +18,8 @@ namespace symhip {
 namespace {
 
 constexpr int kMaxCols = 64;  // supernode width cap (panels padded to this width)
-// Update tile (rows). 32-row tiles on every level, with shared memory sized for them (33 KB, four
-// blocks per CU): a level's update time is the latency of its slowest tiles, and a 64-row tile's
+// Update tile (rows). 32-row tiles on every level, with shared memory sized for them (four
+// blocks per CU at 33 KB, seven with the columns staged in two halves): a level's update time is the latency of its slowest tiles, and a 64-row tile's
 // product (K = 64 on the FP64 matrix cores) and 4096-entry scatter take ~15 us where four 32-row
 // tiles run side by side in ~6 (measured: 1000-camera system 464 -> 388 us of updates per
 // factorization, 100 000-node pose graph 460 -> 205 us).
@@ -323,28 +323,36 @@ __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict
                                             const int32_t* __restrict__ upd_rel, double* smem) {
   static_assert(KT == 64 || KT == 32, "update tiles are 64 or 32 rows");
   constexpr bool kF64 = sizeof(T) == 8;
+  constexpr int kKh = kMaxCols / 2;  // columns staged per half
   auto As = reinterpret_cast<T (*)[KT + 1]>(smem);
-  auto Bs = reinterpret_cast<T (*)[KT + 1]>(reinterpret_cast<T*>(smem) + kMaxCols * (KT + 1));
+  auto Bs = reinterpret_cast<T (*)[KT + 1]>(reinterpret_cast<T*>(smem) + kKh * (KT + 1));
   const SnDesc sd = tk.d;
   const int ti = tk.a, tj = tk.b;
   const int n = sd.ncols, ld = sd.ld, mb = ld - n;
   const double* P = panel + sd.panel_off;
   const double* Ds = D + sd.col_begin;
-  // stage both row tiles (all k at once: the loads are independent)
-  // (unconditional loads from clamped addresses; an update item exists only when mb > 0)
+  // Stage both row tiles, half the columns at a time (unconditional loads from clamped
+  // addresses; an update item exists only when mb > 0). Two halves take half the shared memory
+  // (17 KB: seven blocks per CU instead of four), and a supernode of at most 32 columns skips the
+  // second, all-padding half (the 100 000-node pose graph's updates 206 -> 192 us per
+  // factorization; the 1000-camera system's 63-column supernodes unchanged).
+  auto stage = [&](int h) {
 #pragma unroll
-  for (int t = 0; t < kMaxCols * KT / 256; ++t) {  // blockDim.x == 256
-    const int idx = threadIdx.x + 256 * t;
-    const int k = idx / KT, r = idx % KT;
-    const int ri = ti * KT + r, rj = tj * KT + r;
-    const int kc = min(k, n - 1);
-    const double va = P[(int64_t)kc * ld + n + min(ri, mb - 1)];
-    const double vb = P[(int64_t)kc * ld + n + min(rj, mb - 1)] * Ds[kc];
-    As[k][r] = (k < n && ri < mb) ? T(va) : T(0);
-    Bs[k][r] = (k < n && rj < mb) ? T(vb) : T(0);
-  }
-  // The product on the FP64 matrix cores, in 16 x 16 output blocks, K = 64 in 16 steps of
-  // v_mfma_f64_16x16x4f64: a 64-row tile gives wave w row block w and all four column blocks, a
+    for (int t = 0; t < kKh * KT / 256; ++t) {  // blockDim.x == 256
+      const int idx = threadIdx.x + 256 * t;
+      const int k = idx / KT, r = idx % KT;
+      const int kk = kKh * h + k;
+      const int ri = ti * KT + r, rj = tj * KT + r;
+      const int kc = min(kk, n - 1);
+      const double va = P[(int64_t)kc * ld + n + min(ri, mb - 1)];
+      const double vb = P[(int64_t)kc * ld + n + min(rj, mb - 1)] * Ds[kc];
+      As[k][r] = (kk < n && ri < mb) ? T(va) : T(0);
+      Bs[k][r] = (kk < n && rj < mb) ? T(vb) : T(0);
+    }
+  };
+  stage(0);
+  // The product on the FP64 matrix cores, in 16 x 16 output blocks, K = 64 in two halves of 8
+  // steps of v_mfma_f64_16x16x4f64: a 64-row tile gives wave w row block w and all four column blocks, a
   // 32-row tile gives it row block w / 2 and column block w % 2. Lane l feeds
   // A(i = l % 16, k = l / 16) = As[k][16 rb + i] and B(k, j = l % 16) = Bs[k][16 c + j] and holds
   // D(l / 16 + 4 r, l % 16) of block c in accumulator register r (FP64) or D(4 (l / 16) + r,
@@ -378,18 +386,26 @@ __device__ __forceinline__ void update_body(const SnTask& tk, double* __restrict
   }
   __syncthreads();
   d4 acc[NBW] = {};
+  for (int h = 0; h < 2; ++h) {
+    if (h == 1) {
+      if (n <= kKh) break;  // uniform: the second half is all padding
+      __syncthreads();
+      stage(1);
+      __syncthreads();
+    }
 #pragma unroll 4
-  for (int k0 = 0; k0 < kMaxCols; k0 += 4) {
-    const T av = As[k0 + lk][16 * rb + lr];
-    T bv[NBW];
+    for (int k0 = 0; k0 < kKh; k0 += 4) {
+      const T av = As[k0 + lk][16 * rb + lr];
+      T bv[NBW];
 #pragma unroll
-    for (int c = 0; c < NBW; ++c) bv[c] = Bs[k0 + lk][16 * (c0 + c) + lr];
+      for (int c = 0; c < NBW; ++c) bv[c] = Bs[k0 + lk][16 * (c0 + c) + lr];
 #pragma unroll
-    for (int c = 0; c < NBW; ++c) {
-      if constexpr (kF64)
-        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
-      else
-        acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], acc[c], 0, 0, 0);
+      for (int c = 0; c < NBW; ++c) {
+        if constexpr (kF64)
+          acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[c], 0, 0, 0);
+        else
+          acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[c], acc[c], 0, 0, 0);
+      }
     }
   }
   // transpose through LDS (the tiles' space, as FP64): Tr[column][row]
@@ -419,7 +435,7 @@ __global__ void __launch_bounds__(256) update_kernel(const SnTask* __restrict__ 
                                                      const int64_t* __restrict__ scat_col,
                                                      const int32_t* __restrict__ scat_rel,
                                                      const int32_t* __restrict__ upd_rel) {
-  __shared__ double smem[2 * kMaxCols * (KT + 1)];
+  __shared__ double smem[kMaxCols * (KT + 1)];
   update_body<KT, T>(tasks[blockIdx.x], panel, D, scat_col, scat_rel, upd_rel, smem);
 }
 
